@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Throughput of LEVIATHAN's fused step on one MI355X.
+
+Default workload: Barabasi-Albert graph N=10,000,000, m=10 (heavy-tailed
+in-degree, ~2.0e8 incoming edges), delays uniform in [1, 50], FP32 weights
+and phases.  The step is run with the uint16 exact-wrap history (the headline
+form: ``value``) and with the plain FP32 history (``plain_fp32``), each from
+the same seeded inputs already resident in HBM.
+
+One JSON line on stdout.  ``value`` = edge-steps/s (true nnz x steps / s) of
+the uint16 form; ``hbm_gbs`` = algorithmic bytes per step x steps/s.  The
+``roofline`` object describes the fused step kernel (HIP-event durations,
+measured live); ``cpu_baseline`` times the FP32 OpenMP CPU port of the
+reference's step (oracle/) on a bounded number of steps of the same graph.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ba10m|ws100k|ensemble]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "edge-steps/s + HBM GB/s: uint16-delay/batched vs plain fp32, N=10M k=20"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of one fused step: each logically required byte once.
+#   per edge: col_idx 4 + delay 1 + weight read+write 8 + delayed-phase gather (4 FP32 | 2 uint16)
+#   per node: row_ptr 4 + theta r+w 8 + theta_hat r+w 8 + omega_hat r+w 8 + omega 4 + history write (4 | 2)
+BYTES_PER_EDGE = {"f32": 17, "u16": 15}
+BYTES_PER_NODE = {"f32": 36, "u16": 34}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(N, E, history):
+    return BYTES_PER_EDGE[history] * E + BYTES_PER_NODE[history] * N
+
+
+def make_workload(name, n_override=None):
+    from leviathan_v1_amd import workloads
+
+    if name == "ba10m":
+        n = n_override or 10_000_000
+        return workloads.barabasi_albert_workload(n, 10, 50, seed=0,
+                                                  name=f"barabasi_albert N={n:,} m=10 delays U{{1..50}}")
+    if name == "ws100k":
+        n = n_override or 100_000
+        return workloads.watts_strogatz_workload(n, 20, 0.1, 50, seed=0,
+                                                 name=f"watts_strogatz N={n:,} k=20 p=0.1 delays U{{1..50}}")
+    raise ValueError(name)
+
+
+def run_form(wl, history, steps, warmup, layout="node", ept=8, profile_steps=20):
+    from leviathan_v1_amd import LeviathanEngine
+
+    eng = LeviathanEngine(*wl.engine_args(), history=history, layout=layout, edges_per_thread=ept)
+    try:
+        eng.step_n(wl.dt, warmup)
+        t0 = time.perf_counter()
+        r = eng.step_n(wl.dt, steps)  # returns after the stream is synchronised
+        el = time.perf_counter() - t0
+        prof = eng.profile_steps(wl.dt, profile_steps)
+        theta = eng.get_theta()
+        ok = bool(np.isfinite(theta).all() and np.isfinite(r).all() and 0 <= r.min() and r.max() <= 1)
+        return {
+            "history": history, "layout": layout, "edges_per_thread": ept,
+            "seconds": el, "ms_per_step": 1e3 * el / steps,
+            "edge_steps_per_s": wl.E * steps / el,
+            "hbm_gbs": algorithmic_bytes(wl.N, wl.E, history) * steps / el / 1e9,
+            "step_kernel_ms": prof["step_kernel_ms"], "finalize_kernel_ms": prof["finalize_kernel_ms"],
+            "r_last": float(r[-1]), "invariants_ok": ok, "vram_bytes": eng.get_vram_usage(),
+            "row_blocks": eng.num_blocks,
+        }
+    finally:
+        eng.close()
+
+
+def cpu_baseline(wl, budget_s=15.0, max_steps=50):
+    """FP32 OpenMP port of the reference's four-kernel step (oracle/), same graph and inputs."""
+    import oracle
+
+    oracle.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = oracle.COracle(*wl.engine_args(), dtype=np.float32)
+    o.run(wl.dt, 1)  # first touch
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps and time.perf_counter() - t0 < budget_s:
+        o.run(wl.dt, 1)
+        n += 1
+    el = time.perf_counter() - t0
+    return {"value": wl.E * n / el, "unit": "edge-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the same graph and inputs (FP32 C/OpenMP port of "
+                      f"leviathan_kernel.cu:93-264,391-438), {el:.1f} s"}
+
+
+def load_traffic(workload, history):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(f"{workload}:{history}")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble"])
+    ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
+    ap.add_argument("--layout", default="node", choices=["node", "slot"])
+    ap.add_argument("--ept", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="f32,u16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU "
+              "path: the step shards by node ranges, which is described but not run.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "ensemble":
+        from leviathan_v1_amd import ensemble_bench
+
+        print(json.dumps(ensemble_bench.run(args.steps, args.warmup)))
+        return
+
+    t0 = time.perf_counter()
+    wl = make_workload(args.workload, args.n)
+    log(f"workload {wl.name}: N={wl.N} E={wl.E} built in {time.perf_counter() - t0:.1f} s")
+    forms = {}
+    for h in args.forms.split(","):
+        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept)
+        log(h, json.dumps(forms[h]))
+    head = forms.get("u16") or forms["f32"]
+    hist = head["history"]
+    alg = algorithmic_bytes(wl.N, wl.E, hist)
+    achieved = alg / (head["step_kernel_ms"] * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(wl)
+    if cpu:
+        log("cpu", json.dumps(cpu))
+    out = {
+        "metric": METRIC,
+        "value": head["edge_steps_per_s"],
+        "unit": "edge-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded graph and initial state)",
+        "config": {"workload": wl.name, "N": wl.N, "E": wl.E, "max_delay": wl.max_delay,
+                   "history": hist, "layout": args.layout, "edges_per_thread": args.ept},
+        "hbm_gbs": head["hbm_gbs"],
+        "plain_fp32": forms.get("f32"),
+        "u16": forms.get("u16"),
+        "speedup_u16_vs_fp32": (forms["u16"]["edge_steps_per_s"] / forms["f32"]["edge_steps_per_s"]
+                                if "u16" in forms and "f32" in forms else None),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload, hist),
+                     "kernel": "step_kernel", "algorithmic_bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/* leviathan.h -- C-ABI of the gfx950 LEVIATHAN step library (libleviathan_hip.so).
+ *
+ * The first block of entry points is the reference's own FFI surface: the
+ * extern "C" host API of src/cuda/leviathan_kernel.cu:270-497, which
+ * src/bindings/leviathan_pybind11.cpp:10-25 declares and binds.  Names,
+ * argument order and argument meaning are unchanged, so a caller of the
+ * reference links against this library instead.  Two deliberate differences:
+ *
+ *   - the reference calls exit(1) on any runtime error (leviathan_kernel.cu:11-19);
+ *     here every entry point records a message that leviathan_last_error()
+ *     returns (an empty string after a call that succeeded), init returns
+ *     NULL and step returns NaN on failure;
+ *   - inputs are validated (CSR monotone, columns in range, delays <=
+ *     max_delay) instead of being trusted.
+ *
+ * The second block (leviathan_*_ex, step_n, getters, ensembles, profiling,
+ * reservoir) has no reference counterpart; each says so.
+ *
+ * Threading: a handle is not re-entrant (like the reference's).  Every call
+ * runs on the handle's own HIP stream and returns after that stream has
+ * finished the call's work.  Plain pointers and sizes only: all arrays are
+ * host memory unless a name says otherwise.
+ */
+#ifndef LEVIATHAN_H_
+#define LEVIATHAN_H_
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct LeviathanHandle LeviathanHandle;
+
+/* ---------------------------------------------------------------------------
+ * Reference surface (leviathan_kernel.cu:270-497)
+ * ------------------------------------------------------------------------- */
+
+/* leviathan_kernel.cu:270-389.  Deep-copies the incoming-edge CSR graph
+ * (row_ptr[N+1], col_idx[E], delays[E], E = row_ptr[N]) and the initial state
+ * (weights[E], theta[N], theta_hat[N], omega[N]; omega_hat starts as omega)
+ * to the GPU, with the parameter defaults of :281-297, FP32 history in the
+ * node-major layout.  Returns NULL on failure. */
+LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr,
+                                const int* h_col_idx, const uint8_t* h_delays,
+                                const float* h_weights, const float* h_theta,
+                                const float* h_theta_hat, const float* h_omega);
+
+/* leviathan_kernel.cu:391-438.  One step; returns the order parameter r of the
+ * new phases (NaN on failure).  The gain controller update of :430-435 runs
+ * on the GPU in FP64. */
+float leviathan_step(LeviathanHandle* handle, float dt);
+
+/* leviathan_kernel.cu:440-443.  Copies theta[N] to host memory. */
+void leviathan_get_theta(LeviathanHandle* handle, float* h_theta);
+
+/* leviathan_kernel.cu:445-453.  Uploads theta[N] and re-fills every history
+ * slot with it (the delay memory is erased; theta_hat, omega_hat untouched). */
+void leviathan_set_theta(LeviathanHandle* handle, const float* h_theta);
+
+/* leviathan_kernel.cu:455-478. */
+void leviathan_free(LeviathanHandle* handle);
+
+/* leviathan_kernel.cu:480, :482, :484-486 (network 0 of an ensemble). */
+void leviathan_set_gain(LeviathanHandle* handle, float g);
+float leviathan_get_gain(LeviathanHandle* handle);
+void leviathan_set_gain_controller(LeviathanHandle* handle, bool enabled);
+
+/* leviathan_kernel.cu:488-490.  Bytes of device memory this handle holds
+ * (every allocation counted, unlike :299-310 which misses the partials). */
+size_t leviathan_get_vram_usage(LeviathanHandle* handle);
+
+/* leviathan_kernel.cu:492-497.  Uploads weights[E]. */
+void leviathan_reset_weights(LeviathanHandle* handle, const float* h_weights);
+
+/* ---------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference)
+ * ------------------------------------------------------------------------- */
+
+/* Message of the last failed call on this thread; "" after a success. */
+const char* leviathan_last_error(void);
+
+enum {
+  LEV_HISTORY_F32 = 0, /* history ring stores FP32 phases */
+  LEV_HISTORY_U16 = 1  /* exact-wrap uint16 angles, step 2*pi/65536 */
+};
+enum {
+  LEV_LAYOUT_NODE_MAJOR = 0, /* ring[node][slot] */
+  LEV_LAYOUT_SLOT_MAJOR = 1  /* ring[slot][node] */
+};
+
+typedef struct {
+  /* physics (defaults: leviathan_kernel.cu:281-297, g_rest = 1.5 from :433) */
+  float eta, lam, gamma, alpha_eps, kappa_w, w_min, w_max;
+  float g0, r0, beta, mu, g_rest, g_min, g_max;
+  int32_t gain_controller; /* 0/1 */
+  /* storage and schedule */
+  int32_t history;         /* LEV_HISTORY_* */
+  int32_t layout;          /* LEV_LAYOUT_* */
+  int32_t edges_per_thread;/* 4, 8 or 16: a 256-lane block streams up to 256*this edges */
+  int32_t graph_chunk;     /* steps per captured HIP graph in leviathan_step_n (0: no graphs) */
+} LeviathanConfig;
+
+/* Fills cfg with the defaults above (FP32 node-major history, 8 edges per
+ * thread, 100-step graph chunks). */
+void leviathan_config_default(LeviathanConfig* cfg);
+
+/* Ensemble constructor: n_nets independent networks stored as one
+ * block-diagonal CSR (node ranges net_offsets[n_nets+1], net_offsets[0] = 0,
+ * net_offsets[n_nets] = N; no edge may cross a range).  Per-network initial
+ * gains g0[n_nets] (NULL: cfg->g0 for all) and controller flags
+ * controller[n_nets] (NULL: cfg->gain_controller for all).  All networks
+ * share max_delay, dt and the physics parameters and advance in one fused
+ * launch per step.  n_nets = 1 with net_offsets = {0, N} is leviathan_init. */
+LeviathanHandle* leviathan_init_ex(const LeviathanConfig* cfg, int N, int max_delay,
+                                   const int* h_row_ptr, const int* h_col_idx,
+                                   const uint8_t* h_delays, const float* h_weights,
+                                   const float* h_theta, const float* h_theta_hat,
+                                   const float* h_omega, int n_nets,
+                                   const int* net_offsets, const float* g0,
+                                   const int* controller);
+
+/* n steps with one host synchronisation at the end (HIP-graph chunks when
+ * cfg.graph_chunk > 0).  r_out[n * n_nets] (may be NULL) receives r of
+ * step t for network k at r_out[t * n_nets + k].  Returns 0 or -1. */
+int leviathan_step_n(LeviathanHandle* handle, float dt, int n, float* r_out);
+
+/* Ensemble accessors: per-network gain and controller flag. */
+int leviathan_set_gain_net(LeviathanHandle* handle, int net, float g);
+float leviathan_get_gain_net(LeviathanHandle* handle, int net);
+int leviathan_set_gain_controller_net(LeviathanHandle* handle, int net, bool enabled);
+int leviathan_num_nets(LeviathanHandle* handle);
+
+/* State export: weights[E]; theta_hat[N] and omega_hat[N]; theta(t - delay)
+ * for every node as stored in the history ring (decoded to radians), for
+ * 0 <= delay <= max_delay.  Return 0 or -1. */
+int leviathan_get_weights(LeviathanHandle* handle, float* h_weights);
+int leviathan_get_predictor(LeviathanHandle* handle, float* h_theta_hat, float* h_omega_hat);
+int leviathan_get_history(LeviathanHandle* handle, int delay, float* h_out);
+
+/* Sizes and the number of steps taken so far. */
+int leviathan_num_nodes(LeviathanHandle* handle);
+int64_t leviathan_num_edges(LeviathanHandle* handle);
+int64_t leviathan_step_count(LeviathanHandle* handle);
+int leviathan_num_blocks(LeviathanHandle* handle);
+
+/* Runs n eager steps and times them with HIP events on the handle's stream.
+ * out[0] = mean duration of the fused step kernel (ms per launch),
+ * out[1] = mean duration of the per-step finalize kernel (ms per launch),
+ * out[2] = wall time of the n steps between the first and last event (ms).
+ * Returns 0 or -1. */
+int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* out);
+
+/* Input drive (reservoir): omega_i(t) = omega_i + drive_b[i] * u_t.
+ * leviathan_set_drive uploads drive_b[N] (NULL switches the drive off);
+ * leviathan_step_n_driven runs n steps with u_t = u[t]. */
+int leviathan_set_drive(LeviathanHandle* handle, const float* h_drive_b);
+int leviathan_step_n_driven(LeviathanHandle* handle, float dt, int n, const float* h_u, float* r_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LEVIATHAN_H_ */

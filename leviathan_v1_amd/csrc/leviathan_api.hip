@@ -1,0 +1,830 @@
+// leviathan_api.hip -- host side of the C-ABI declared in include/leviathan.h.
+//
+// Reference counterpart: the host API of src/cuda/leviathan_kernel.cu:266-497
+// (LeviathanHandle, leviathan_init/step/get_theta/set_theta/free/set_gain/
+// get_gain/set_gain_controller/get_vram_usage/reset_weights).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "leviathan.h"
+#include "leviathan_internal.h"
+
+using lev::FinalizeArgs;
+using lev::NetCtrl;
+using lev::StepArgs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_err(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+#define LEV_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      set_err("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return false;                                                                        \
+    }                                                                                      \
+  } while (0)
+
+struct GraphEntry {
+  uint32_t dt_bits;
+  int chunk;
+  hipGraphExec_t exec;
+};
+
+}  // namespace
+
+struct LeviathanHandle {
+  LeviathanConfig cfg{};
+  int device = 0;
+  int32_t N = 0;
+  int64_t E = 0;
+  int32_t max_delay = 0;
+  int32_t n_slots = 0;      // S = max_delay + 2
+  int64_t ring_pitch = 0;
+  int32_t n_nets = 1;
+  int32_t num_blocks = 0;
+  int64_t step = 0;         // host mirror of the device step counter
+  size_t vram = 0;
+
+  int32_t* d_row_ptr = nullptr;
+  int32_t* d_col = nullptr;
+  uint8_t* d_delay = nullptr;
+  float* d_w = nullptr;
+  float* d_theta = nullptr;
+  float* d_theta_hat = nullptr;
+  float* d_omega = nullptr;
+  float* d_omega_hat = nullptr;
+  void* d_ring = nullptr;
+  int4* d_blocks = nullptr;
+  double2* d_partials = nullptr;
+  NetCtrl* d_ctrl = nullptr;
+  int64_t* d_step = nullptr;
+  float* d_r_hist = nullptr;
+  float* d_scratch = nullptr;   // N floats (history export)
+  float* d_drive_b = nullptr;
+  float* d_drive_u = nullptr;
+  bool drive_on = false;
+
+  float* h_pinned = nullptr;    // kRHistSteps * n_nets floats
+  hipStream_t stream = nullptr;
+  std::vector<NetCtrl> h_ctrl;  // host copy of the static controller fields
+  std::vector<GraphEntry> graphs;
+};
+
+namespace {
+
+template <class T>
+bool dalloc(LeviathanHandle* h, T** p, size_t count) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    set_err("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    *p = nullptr;
+    return false;
+  }
+  h->vram += bytes;
+  return true;
+}
+
+void drop_graphs(LeviathanHandle* h) {
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+}
+
+void destroy(LeviathanHandle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graphs(h);
+  void* ptrs[] = {h->d_row_ptr, h->d_col,   h->d_delay,  h->d_w,      h->d_theta,   h->d_theta_hat,
+                  h->d_omega,   h->d_omega_hat, h->d_ring, h->d_blocks, h->d_partials, h->d_ctrl,
+                  h->d_step,    h->d_r_hist, h->d_scratch, h->d_drive_b, h->d_drive_u};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+bool check_handle(LeviathanHandle* h) {
+  g_last_error.clear();
+  if (!h) {
+    set_err("null LeviathanHandle");
+    return false;
+  }
+  if (hipSetDevice(h->device) != hipSuccess) {
+    set_err("hipSetDevice(%d) failed", h->device);
+    return false;
+  }
+  return true;
+}
+
+// Row blocks: runs of <= 256 rows with <= cap edges, or one long row.
+// Blocks of a network are contiguous; within a network long rows come first
+// (heaviest first) so they start early.
+bool build_blocks(const int* row_ptr, int n_nets, const int* net_offsets, int cap,
+                  std::vector<int4>& blocks, std::vector<NetCtrl>& ctrl) {
+  blocks.clear();
+  for (int net = 0; net < n_nets; ++net) {
+    const int lo = net_offsets[net], hi = net_offsets[net + 1];
+    std::vector<int4> longs, shorts;
+    int r = lo;
+    while (r < hi) {
+      const int deg = row_ptr[r + 1] - row_ptr[r];
+      if (deg > cap) {
+        longs.push_back(make_int4(r, r + 1, net, lev::kBlockLongRow));
+        ++r;
+        continue;
+      }
+      const int start = r;
+      int nnz = 0;
+      while (r < hi && r - start < lev::kBlock) {
+        const int d = row_ptr[r + 1] - row_ptr[r];
+        if (d > cap || nnz + d > cap) break;
+        nnz += d;
+        ++r;
+      }
+      shorts.push_back(make_int4(start, r, net, 0));
+    }
+    std::stable_sort(longs.begin(), longs.end(), [&](const int4& a, const int4& b) {
+      return (row_ptr[a.x + 1] - row_ptr[a.x]) > (row_ptr[b.x + 1] - row_ptr[b.x]);
+    });
+    ctrl[net].blk_begin = (int32_t)blocks.size();
+    blocks.insert(blocks.end(), longs.begin(), longs.end());
+    blocks.insert(blocks.end(), shorts.begin(), shorts.end());
+    ctrl[net].blk_end = (int32_t)blocks.size();
+  }
+  if (blocks.size() > (size_t)std::numeric_limits<int32_t>::max()) {
+    set_err("too many row blocks");
+    return false;
+  }
+  return true;
+}
+
+StepArgs step_args(LeviathanHandle* h, float dt) {
+  StepArgs a{};
+  a.row_ptr = h->d_row_ptr;
+  a.col = h->d_col;
+  a.delay = h->d_delay;
+  a.w = h->d_w;
+  a.theta = h->d_theta;
+  a.theta_hat = h->d_theta_hat;
+  a.omega = h->d_omega;
+  a.omega_hat = h->d_omega_hat;
+  a.ring = h->d_ring;
+  a.ring_pitch = h->ring_pitch;
+  a.n_slots = h->n_slots;
+  a.num_blocks = h->num_blocks;
+  a.xcd_remap = 1;
+  a.blocks = h->d_blocks;
+  a.partials = h->d_partials;
+  a.ctrl = h->d_ctrl;
+  a.step = h->d_step;
+  a.drive_b = h->drive_on ? h->d_drive_b : nullptr;
+  a.drive_u = h->d_drive_u;
+  a.dt = dt;
+  a.eta = h->cfg.eta;
+  a.lam = h->cfg.lam;
+  a.gamma = h->cfg.gamma;
+  a.alpha = h->cfg.alpha_eps;
+  a.kappa_w = h->cfg.kappa_w;
+  a.w_min = h->cfg.w_min;
+  a.w_max = h->cfg.w_max;
+  return a;
+}
+
+FinalizeArgs fin_args(LeviathanHandle* h, float dt) {
+  FinalizeArgs f{};
+  f.partials = h->d_partials;
+  f.ctrl = h->d_ctrl;
+  f.step = h->d_step;
+  f.r_hist = h->d_r_hist;
+  f.n_nets = h->n_nets;
+  f.num_blocks = h->num_blocks;
+  f.dt = dt;
+  return f;
+}
+
+// Enqueue one step (two launches) on the handle's stream.
+bool enqueue_step(LeviathanHandle* h, float dt) {
+  const StepArgs a = step_args(h, dt);
+  LEV_HIP(lev::launch_step(h->cfg.history, h->cfg.layout, h->cfg.edges_per_thread, a, h->stream));
+  LEV_HIP(lev::launch_finalize(fin_args(h, dt), h->stream));
+  return true;
+}
+
+bool get_graph(LeviathanHandle* h, float dt, int chunk, hipGraphExec_t* out) {
+  uint32_t bits;
+  std::memcpy(&bits, &dt, 4);
+  for (auto& g : h->graphs)
+    if (g.dt_bits == bits && g.chunk == chunk) {
+      *out = g.exec;
+      return true;
+    }
+  hipGraph_t graph = nullptr;
+  LEV_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  bool ok = true;
+  for (int c = 0; c < chunk && ok; ++c) ok = enqueue_step(h, dt);
+  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (!ok) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return false;
+  }
+  if (e != hipSuccess) {
+    set_err("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return false;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    set_err("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    return false;
+  }
+  if (h->graphs.size() >= 8) drop_graphs(h);
+  h->graphs.push_back({bits, chunk, exec});
+  *out = exec;
+  return true;
+}
+
+// Run n (<= kRHistSteps) steps and copy their r values out.
+bool run_segment(LeviathanHandle* h, float dt, int n, float* r_out) {
+  const int chunk = h->cfg.graph_chunk;
+  int done = 0;
+  if (chunk > 0) {
+    while (n - done >= chunk) {
+      hipGraphExec_t exec;
+      if (!get_graph(h, dt, chunk, &exec)) return false;
+      LEV_HIP(hipGraphLaunch(exec, h->stream));
+      done += chunk;
+    }
+  }
+  for (; done < n; ++done)
+    if (!enqueue_step(h, dt)) return false;
+  const int64_t first = h->step;
+  h->step += n;
+  if (r_out) {
+    const int nn = h->n_nets;
+    int64_t slot = first % lev::kRHistSteps;
+    int left = n, off = 0;
+    while (left > 0) {
+      const int run = (int)std::min<int64_t>(left, lev::kRHistSteps - slot);
+      LEV_HIP(hipMemcpyAsync(h->h_pinned + (size_t)off * nn, h->d_r_hist + (size_t)slot * nn,
+                             sizeof(float) * (size_t)run * nn, hipMemcpyDeviceToHost, h->stream));
+      left -= run;
+      off += run;
+      slot = 0;
+    }
+  }
+  LEV_HIP(hipStreamSynchronize(h->stream));
+  if (r_out) std::memcpy(r_out, h->h_pinned, sizeof(float) * (size_t)n * h->n_nets);
+  return true;
+}
+
+bool upload_drive_u(LeviathanHandle* h, const float* u, int n) {
+  int64_t slot = h->step % lev::kDriveSteps;
+  int left = n, off = 0;
+  while (left > 0) {
+    const int run = (int)std::min<int64_t>(left, lev::kDriveSteps - slot);
+    LEV_HIP(hipMemcpyAsync(h->d_drive_u + slot, u + off, sizeof(float) * run, hipMemcpyHostToDevice,
+                           h->stream));
+    left -= run;
+    off += run;
+    slot = 0;
+  }
+  return true;
+}
+
+bool validate_inputs(int N, int max_delay, const int* row_ptr, const int* col_idx, const uint8_t* delays,
+                     const float* weights, const float* theta, const float* theta_hat, const float* omega) {
+  if (N <= 0) {
+    set_err("N must be positive (got %d)", N);
+    return false;
+  }
+  if (max_delay < 0 || max_delay > 255) {
+    set_err("max_delay must be in [0, 255] (delays are uint8), got %d", max_delay);
+    return false;
+  }
+  if (!row_ptr || !theta || !theta_hat || !omega) {
+    set_err("null input array");
+    return false;
+  }
+  if (row_ptr[0] != 0) {
+    set_err("row_ptr[0] must be 0");
+    return false;
+  }
+  for (int i = 0; i < N; ++i)
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      set_err("row_ptr decreases at row %d", i);
+      return false;
+    }
+  const int64_t E = row_ptr[N];
+  if (E > 0 && (!col_idx || !delays || !weights)) {
+    set_err("null edge array");
+    return false;
+  }
+  for (int64_t e = 0; e < E; ++e) {
+    if (col_idx[e] < 0 || col_idx[e] >= N) {
+      set_err("col_idx[%lld] = %d out of range [0, %d)", (long long)e, col_idx[e], N);
+      return false;
+    }
+    if (delays[e] > max_delay) {
+      set_err("delays[%lld] = %d exceeds max_delay %d", (long long)e, (int)delays[e], max_delay);
+      return false;
+    }
+  }
+  return true;
+}
+
+LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, const int* row_ptr,
+                        const int* col_idx, const uint8_t* delays, const float* weights, const float* theta,
+                        const float* theta_hat, const float* omega, int n_nets, const int* net_offsets,
+                        const float* g0, const int* controller) {
+  g_last_error.clear();
+  LeviathanConfig cfg;
+  if (cfg_in)
+    cfg = *cfg_in;
+  else
+    leviathan_config_default(&cfg);
+  if (cfg.history != LEV_HISTORY_F32 && cfg.history != LEV_HISTORY_U16) {
+    set_err("unknown history format %d", cfg.history);
+    return nullptr;
+  }
+  if (cfg.layout != LEV_LAYOUT_NODE_MAJOR && cfg.layout != LEV_LAYOUT_SLOT_MAJOR) {
+    set_err("unknown history layout %d", cfg.layout);
+    return nullptr;
+  }
+  if (cfg.edges_per_thread != 4 && cfg.edges_per_thread != 8 && cfg.edges_per_thread != 16) {
+    set_err("edges_per_thread must be 4, 8 or 16 (got %d)", cfg.edges_per_thread);
+    return nullptr;
+  }
+  if (cfg.graph_chunk < 0 || cfg.graph_chunk > lev::kRHistSteps) {
+    set_err("graph_chunk must be in [0, %d]", lev::kRHistSteps);
+    return nullptr;
+  }
+  if (!validate_inputs(N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega)) return nullptr;
+  std::vector<int> offs;
+  if (n_nets <= 0) {
+    set_err("n_nets must be positive");
+    return nullptr;
+  }
+  if (net_offsets) {
+    offs.assign(net_offsets, net_offsets + n_nets + 1);
+  } else {
+    if (n_nets != 1) {
+      set_err("net_offsets is required when n_nets > 1");
+      return nullptr;
+    }
+    offs = {0, N};
+  }
+  if (offs[0] != 0 || offs[n_nets] != N) {
+    set_err("net_offsets must start at 0 and end at N");
+    return nullptr;
+  }
+  for (int k = 0; k < n_nets; ++k) {
+    if (offs[k + 1] <= offs[k]) {
+      set_err("network %d has no nodes", k);
+      return nullptr;
+    }
+    for (int i = offs[k]; i < offs[k + 1]; ++i)
+      for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+        if (col_idx[e] < offs[k] || col_idx[e] >= offs[k + 1]) {
+          set_err("edge %d of row %d crosses from network %d", e, i, k);
+          return nullptr;
+        }
+  }
+
+  LeviathanHandle* h = new LeviathanHandle();
+  h->cfg = cfg;
+  h->N = N;
+  h->E = row_ptr[N];
+  h->max_delay = max_delay;
+  h->n_slots = max_delay + 2;
+  h->n_nets = n_nets;
+  auto fail = [&]() -> LeviathanHandle* {
+    std::string msg = g_last_error;
+    destroy(h);
+    g_last_error = msg;
+    return nullptr;
+  };
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+      set_err("%s failed: %s", what, hipGetErrorString(e));
+      return false;
+    }
+    return true;
+  };
+  if (!hip_ok(hipGetDevice(&h->device), "hipGetDevice")) return fail();
+  if (!hip_ok(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return fail();
+
+  const size_t hsize = cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+  if (cfg.layout == LEV_LAYOUT_NODE_MAJOR) {
+    const int64_t per_line = 128 / (int64_t)hsize;  // pad each node's slots to whole 128-byte lines
+    h->ring_pitch = ((h->n_slots + per_line - 1) / per_line) * per_line;
+    if (h->n_slots <= per_line / 2) h->ring_pitch = h->n_slots;  // tiny rings: no padding
+  } else {
+    h->ring_pitch = ((int64_t)N + 63) / 64 * 64;
+  }
+  const size_t ring_elems = cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)N * h->ring_pitch
+                                                               : (size_t)h->n_slots * h->ring_pitch;
+
+  std::vector<int4> blocks;
+  h->h_ctrl.assign(n_nets, NetCtrl{});
+  if (!build_blocks(row_ptr, n_nets, offs.data(), lev::kBlock * cfg.edges_per_thread, blocks, h->h_ctrl))
+    return fail();
+  h->num_blocks = (int32_t)blocks.size();
+  for (int k = 0; k < n_nets; ++k) {
+    NetCtrl& c = h->h_ctrl[k];
+    c.g = g0 ? (double)g0[k] : (double)cfg.g0;
+    c.r = 0.0;
+    c.r0 = cfg.r0;
+    c.beta = cfg.beta;
+    c.mu = cfg.mu;
+    c.g_rest = cfg.g_rest;
+    c.g_min = cfg.g_min;
+    c.g_max = cfg.g_max;
+    c.controller = controller ? (controller[k] != 0) : (cfg.gain_controller != 0);
+    c.n_nodes = offs[k + 1] - offs[k];
+  }
+
+  if (!dalloc(h, &h->d_row_ptr, (size_t)N + 1) || !dalloc(h, &h->d_col, (size_t)h->E) ||
+      !dalloc(h, &h->d_delay, (size_t)h->E) || !dalloc(h, &h->d_w, (size_t)h->E) ||
+      !dalloc(h, &h->d_theta, (size_t)N) || !dalloc(h, &h->d_theta_hat, (size_t)N) ||
+      !dalloc(h, &h->d_omega, (size_t)N) || !dalloc(h, &h->d_omega_hat, (size_t)N) ||
+      !dalloc(h, (char**)&h->d_ring, ring_elems * hsize) || !dalloc(h, &h->d_blocks, blocks.size()) ||
+      !dalloc(h, &h->d_partials, blocks.size()) || !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
+      !dalloc(h, &h->d_step, 1) || !dalloc(h, &h->d_r_hist, (size_t)lev::kRHistSteps * n_nets) ||
+      !dalloc(h, &h->d_scratch, (size_t)N) || !dalloc(h, &h->d_drive_u, (size_t)lev::kDriveSteps))
+    return fail();
+  if (!hip_ok(hipHostMalloc((void**)&h->h_pinned, sizeof(float) * (size_t)lev::kRHistSteps * n_nets,
+                            hipHostMallocDefault),
+              "hipHostMalloc"))
+    return fail();
+
+  const hipMemcpyKind H2D = hipMemcpyHostToDevice;
+  const int64_t zero = 0;
+  bool ok = hip_ok(hipMemcpy(h->d_row_ptr, row_ptr, sizeof(int) * ((size_t)N + 1), H2D), "copy row_ptr") &&
+            (h->E == 0 || (hip_ok(hipMemcpy(h->d_col, col_idx, sizeof(int) * h->E, H2D), "copy col_idx") &&
+                           hip_ok(hipMemcpy(h->d_delay, delays, h->E, H2D), "copy delays") &&
+                           hip_ok(hipMemcpy(h->d_w, weights, sizeof(float) * h->E, H2D), "copy weights"))) &&
+            hip_ok(hipMemcpy(h->d_theta, theta, sizeof(float) * N, H2D), "copy theta") &&
+            hip_ok(hipMemcpy(h->d_theta_hat, theta_hat, sizeof(float) * N, H2D), "copy theta_hat") &&
+            hip_ok(hipMemcpy(h->d_omega, omega, sizeof(float) * N, H2D), "copy omega") &&
+            // omega_hat starts as omega (leviathan_kernel.cu:350)
+            hip_ok(hipMemcpy(h->d_omega_hat, omega, sizeof(float) * N, H2D), "copy omega_hat") &&
+            hip_ok(hipMemcpy(h->d_blocks, blocks.data(), sizeof(int4) * blocks.size(), H2D), "copy blocks") &&
+            hip_ok(hipMemcpy(h->d_ctrl, h->h_ctrl.data(), sizeof(NetCtrl) * n_nets, H2D), "copy ctrl") &&
+            hip_ok(hipMemcpy(h->d_step, &zero, sizeof(int64_t), H2D), "copy step") &&
+            hip_ok(hipMemset(h->d_r_hist, 0, sizeof(float) * (size_t)lev::kRHistSteps * n_nets), "memset r") &&
+            hip_ok(hipMemset(h->d_drive_u, 0, sizeof(float) * lev::kDriveSteps), "memset u") &&
+            hip_ok(hipMemset(h->d_partials, 0, sizeof(double2) * blocks.size()), "memset partials");
+  if (!ok) return fail();
+  // history pre-fill (init_buffer_kernel, leviathan_kernel.cu:371-372)
+  if (!hip_ok(lev::launch_fill_ring(cfg.history, cfg.layout, h->d_ring, h->ring_pitch, h->n_slots, h->d_theta, N,
+                                    h->stream),
+              "fill history") ||
+      !hip_ok(hipStreamSynchronize(h->stream), "hipStreamSynchronize"))
+    return fail();
+  return h;
+}
+
+bool refresh_ctrl_field(LeviathanHandle* h, int net, size_t offset, const void* src, size_t bytes) {
+  LEV_HIP(hipStreamSynchronize(h->stream));
+  LEV_HIP(hipMemcpy(reinterpret_cast<char*>(h->d_ctrl + net) + offset, src, bytes, hipMemcpyHostToDevice));
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* leviathan_last_error(void) { return g_last_error.c_str(); }
+
+void leviathan_config_default(LeviathanConfig* c) {
+  if (!c) return;
+  // leviathan_kernel.cu:281-297; g_rest is the literal 1.5f of :433
+  c->eta = 0.01f;
+  c->lam = 0.001f;
+  c->gamma = 0.05f;
+  c->alpha_eps = 0.1f;
+  c->kappa_w = 0.01f;
+  c->w_min = 0.0f;
+  c->w_max = 2.0f;
+  c->g0 = 1.5f;
+  c->r0 = 0.5f;
+  c->beta = 0.01f;
+  c->mu = 0.01f;
+  c->g_rest = 1.5f;
+  c->g_min = 0.1f;
+  c->g_max = 5.0f;
+  c->gain_controller = 1;
+  c->history = LEV_HISTORY_F32;
+  c->layout = LEV_LAYOUT_NODE_MAJOR;
+  c->edges_per_thread = 8;
+  c->graph_chunk = 100;
+}
+
+LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr, const int* h_col_idx,
+                                const uint8_t* h_delays, const float* h_weights, const float* h_theta,
+                                const float* h_theta_hat, const float* h_omega) {
+  return create(nullptr, N, max_delay, h_row_ptr, h_col_idx, h_delays, h_weights, h_theta, h_theta_hat, h_omega, 1,
+                nullptr, nullptr, nullptr);
+}
+
+LeviathanHandle* leviathan_init_ex(const LeviathanConfig* cfg, int N, int max_delay, const int* h_row_ptr,
+                                   const int* h_col_idx, const uint8_t* h_delays, const float* h_weights,
+                                   const float* h_theta, const float* h_theta_hat, const float* h_omega, int n_nets,
+                                   const int* net_offsets, const float* g0, const int* controller) {
+  return create(cfg, N, max_delay, h_row_ptr, h_col_idx, h_delays, h_weights, h_theta, h_theta_hat, h_omega, n_nets,
+                net_offsets, g0, controller);
+}
+
+float leviathan_step(LeviathanHandle* h, float dt) {
+  if (!check_handle(h)) return std::numeric_limits<float>::quiet_NaN();
+  if (h->drive_on) {
+    // a drive without an input sequence holds u = 0
+    const float zero = 0.0f;
+    if (!upload_drive_u(h, &zero, 1)) return std::numeric_limits<float>::quiet_NaN();
+  }
+  const int64_t slot = h->step % lev::kRHistSteps;
+  if (!enqueue_step(h, dt)) return std::numeric_limits<float>::quiet_NaN();
+  hipError_t e = hipMemcpyAsync(h->h_pinned, h->d_r_hist + slot * h->n_nets, sizeof(float), hipMemcpyDeviceToHost,
+                                h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_err("leviathan_step: %s", hipGetErrorString(e));
+    return std::numeric_limits<float>::quiet_NaN();
+  }
+  h->step += 1;
+  return h->h_pinned[0];
+}
+
+int leviathan_step_n(LeviathanHandle* h, float dt, int n, float* r_out) {
+  if (!check_handle(h)) return -1;
+  if (n < 0) {
+    set_err("n must be >= 0");
+    return -1;
+  }
+  const int seg_max = std::min(lev::kRHistSteps, lev::kDriveSteps);
+  std::vector<float> zeros;
+  for (int done = 0; done < n;) {
+    const int seg = std::min(n - done, seg_max);
+    if (h->drive_on) {
+      zeros.assign(seg, 0.0f);
+      if (!upload_drive_u(h, zeros.data(), seg)) return -1;
+    }
+    if (!run_segment(h, dt, seg, r_out ? r_out + (size_t)done * h->n_nets : nullptr)) return -1;
+    done += seg;
+  }
+  return 0;
+}
+
+int leviathan_step_n_driven(LeviathanHandle* h, float dt, int n, const float* u, float* r_out) {
+  if (!check_handle(h)) return -1;
+  if (!h->drive_on) {
+    set_err("no drive set: call leviathan_set_drive first");
+    return -1;
+  }
+  if (n < 0 || (n > 0 && !u)) {
+    set_err("need n >= 0 and an input sequence");
+    return -1;
+  }
+  const int seg_max = std::min(lev::kRHistSteps, lev::kDriveSteps);
+  for (int done = 0; done < n;) {
+    const int seg = std::min(n - done, seg_max);
+    if (!upload_drive_u(h, u + done, seg)) return -1;
+    if (!run_segment(h, dt, seg, r_out ? r_out + (size_t)done * h->n_nets : nullptr)) return -1;
+    done += seg;
+  }
+  return 0;
+}
+
+int leviathan_set_drive(LeviathanHandle* h, const float* b) {
+  if (!check_handle(h)) return -1;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_err("hipStreamSynchronize failed");
+    return -1;
+  }
+  drop_graphs(h);  // captured graphs hold the drive pointer
+  if (!b) {
+    h->drive_on = false;
+    return 0;
+  }
+  if (!h->d_drive_b && !dalloc(h, &h->d_drive_b, (size_t)h->N)) return -1;
+  hipError_t e = hipMemcpy(h->d_drive_b, b, sizeof(float) * h->N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err("copy drive: %s", hipGetErrorString(e));
+    return -1;
+  }
+  h->drive_on = true;
+  return 0;
+}
+
+void leviathan_get_theta(LeviathanHandle* h, float* out) {
+  if (!check_handle(h)) return;
+  if (!out) {
+    set_err("null output array");
+    return;
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, h->d_theta, sizeof(float) * h->N, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) set_err("leviathan_get_theta: %s", hipGetErrorString(e));
+}
+
+void leviathan_set_theta(LeviathanHandle* h, const float* in) {
+  if (!check_handle(h)) return;
+  if (!in) {
+    set_err("null input array");
+    return;
+  }
+  hipError_t e = hipMemcpyAsync(h->d_theta, in, sizeof(float) * h->N, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = lev::launch_fill_ring(h->cfg.history, h->cfg.layout, h->d_ring, h->ring_pitch, h->n_slots, h->d_theta, h->N,
+                              h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) set_err("leviathan_set_theta: %s", hipGetErrorString(e));
+}
+
+void leviathan_free(LeviathanHandle* h) {
+  g_last_error.clear();
+  destroy(h);
+}
+
+int leviathan_set_gain_net(LeviathanHandle* h, int net, float g) {
+  if (!check_handle(h)) return -1;
+  if (net < 0 || net >= h->n_nets) {
+    set_err("network %d out of range", net);
+    return -1;
+  }
+  const double gd = g;
+  return refresh_ctrl_field(h, net, offsetof(NetCtrl, g), &gd, sizeof(double)) ? 0 : -1;
+}
+
+float leviathan_get_gain_net(LeviathanHandle* h, int net) {
+  if (!check_handle(h)) return std::numeric_limits<float>::quiet_NaN();
+  if (net < 0 || net >= h->n_nets) {
+    set_err("network %d out of range", net);
+    return std::numeric_limits<float>::quiet_NaN();
+  }
+  double g = 0.0;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(&g, reinterpret_cast<char*>(h->d_ctrl + net) + offsetof(NetCtrl, g), sizeof(double),
+                  hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("get_gain: %s", hipGetErrorString(e));
+    return std::numeric_limits<float>::quiet_NaN();
+  }
+  return (float)g;
+}
+
+int leviathan_set_gain_controller_net(LeviathanHandle* h, int net, bool enabled) {
+  if (!check_handle(h)) return -1;
+  if (net < 0 || net >= h->n_nets) {
+    set_err("network %d out of range", net);
+    return -1;
+  }
+  const int32_t v = enabled ? 1 : 0;
+  return refresh_ctrl_field(h, net, offsetof(NetCtrl, controller), &v, sizeof(int32_t)) ? 0 : -1;
+}
+
+void leviathan_set_gain(LeviathanHandle* h, float g) { leviathan_set_gain_net(h, 0, g); }
+
+float leviathan_get_gain(LeviathanHandle* h) { return leviathan_get_gain_net(h, 0); }
+
+void leviathan_set_gain_controller(LeviathanHandle* h, bool enabled) {
+  leviathan_set_gain_controller_net(h, 0, enabled);
+}
+
+size_t leviathan_get_vram_usage(LeviathanHandle* h) {
+  if (!check_handle(h)) return 0;
+  return h->vram;
+}
+
+void leviathan_reset_weights(LeviathanHandle* h, const float* w) {
+  if (!check_handle(h)) return;
+  if (!w && h->E > 0) {
+    set_err("null weights array");
+    return;
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && h->E > 0) e = hipMemcpy(h->d_w, w, sizeof(float) * h->E, hipMemcpyHostToDevice);
+  if (e != hipSuccess) set_err("leviathan_reset_weights: %s", hipGetErrorString(e));
+}
+
+int leviathan_get_weights(LeviathanHandle* h, float* out) {
+  if (!check_handle(h)) return -1;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && h->E > 0) e = hipMemcpy(out, h->d_w, sizeof(float) * h->E, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("get_weights: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_get_predictor(LeviathanHandle* h, float* theta_hat, float* omega_hat) {
+  if (!check_handle(h)) return -1;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && theta_hat)
+    e = hipMemcpy(theta_hat, h->d_theta_hat, sizeof(float) * h->N, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && omega_hat)
+    e = hipMemcpy(omega_hat, h->d_omega_hat, sizeof(float) * h->N, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("get_predictor: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_get_history(LeviathanHandle* h, int delay, float* out) {
+  if (!check_handle(h)) return -1;
+  if (delay < 0 || delay > h->max_delay) {
+    set_err("delay %d out of [0, %d]", delay, h->max_delay);
+    return -1;
+  }
+  const int32_t slot = (int32_t)(((h->step - delay) % h->n_slots + h->n_slots) % h->n_slots);
+  hipError_t e = lev::launch_read_slot(h->cfg.history, h->cfg.layout, h->d_ring, h->ring_pitch, slot, h->d_scratch,
+                                       h->N, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, h->d_scratch, sizeof(float) * h->N, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("get_history: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_num_nodes(LeviathanHandle* h) { return h ? h->N : 0; }
+int64_t leviathan_num_edges(LeviathanHandle* h) { return h ? h->E : 0; }
+int64_t leviathan_step_count(LeviathanHandle* h) { return h ? h->step : 0; }
+int leviathan_num_nets(LeviathanHandle* h) { return h ? h->n_nets : 0; }
+int leviathan_num_blocks(LeviathanHandle* h) { return h ? h->num_blocks : 0; }
+
+int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
+  if (!check_handle(h)) return -1;
+  if (n <= 0 || !out) {
+    set_err("need n > 0 and an output array");
+    return -1;
+  }
+  if (h->drive_on) {
+    set_err("profile_steps runs undriven steps; switch the drive off first");
+    return -1;
+  }
+  std::vector<hipEvent_t> ev((size_t)2 * n + 1, nullptr);
+  bool ok = true;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) ok = false;
+  if (ok) {
+    const StepArgs a = step_args(h, dt);
+    const FinalizeArgs f = fin_args(h, dt);
+    ok = hipEventRecord(ev[0], h->stream) == hipSuccess;
+    for (int t = 0; t < n && ok; ++t) {
+      ok = lev::launch_step(h->cfg.history, h->cfg.layout, h->cfg.edges_per_thread, a, h->stream) == hipSuccess &&
+           hipEventRecord(ev[2 * t + 1], h->stream) == hipSuccess &&
+           lev::launch_finalize(f, h->stream) == hipSuccess && hipEventRecord(ev[2 * t + 2], h->stream) == hipSuccess;
+    }
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+  }
+  if (ok) {
+    double ks = 0.0, kf = 0.0;
+    for (int t = 0; t < n; ++t) {
+      float a = 0.f, b = 0.f;
+      (void)hipEventElapsedTime(&a, ev[2 * t], ev[2 * t + 1]);
+      (void)hipEventElapsedTime(&b, ev[2 * t + 1], ev[2 * t + 2]);
+      ks += a;
+      kf += b;
+    }
+    float wall = 0.f;
+    (void)hipEventElapsedTime(&wall, ev[0], ev[2 * n]);
+    out[0] = ks / n;
+    out[1] = kf / n;
+    out[2] = wall;
+    h->step += n;
+  } else {
+    set_err("profile_steps: a HIP call failed");
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return ok ? 0 : -1;
+}
+
+}  // extern "C"

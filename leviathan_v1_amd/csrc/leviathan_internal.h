@@ -1,0 +1,78 @@
+// leviathan_internal.h -- device-side data structures shared by the HIP
+// kernels (leviathan_kernels.hip) and the host C-ABI (leviathan_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lev {
+
+constexpr int kBlock = 256;          // lanes per step workgroup (4 waves of 64)
+constexpr int kFinalizeThreads = 1024;
+constexpr int kRHistSteps = 4096;    // steps of r kept on the device ring
+constexpr int kDriveSteps = 4096;    // steps of drive input u_t kept on the device ring
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Row block flag: a single row whose in-degree exceeds the block capacity.
+constexpr int kBlockLongRow = 1;
+
+// Per-network controller state, read by the step kernel (g) and updated by
+// the finalize kernel (r, g) in FP64.
+struct NetCtrl {
+  double g;        // gain used by the next step
+  double r;        // r of the last step
+  double r0, beta, mu, g_rest, g_min, g_max;
+  int32_t controller;
+  int32_t n_nodes;
+  int32_t blk_begin;  // range of row blocks (and partials) of this network
+  int32_t blk_end;
+};
+
+template <class H>
+struct RingRef {
+  H* ring;
+  int64_t pitch;  // node-major: elements per node; slot-major: elements per slot
+};
+
+struct StepArgs {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const uint8_t* delay;
+  float* w;
+  float* theta;
+  float* theta_hat;
+  const float* omega;
+  float* omega_hat;
+  void* ring;
+  int64_t ring_pitch;
+  int32_t n_slots;        // S = max_delay + 2
+  int32_t num_blocks;
+  int32_t xcd_remap;
+  const int4* blocks;     // {row_begin, row_end, net, flags}
+  double2* partials;      // per row block: (sum cos, sum sin) of the new phases
+  const NetCtrl* ctrl;
+  const int64_t* step;    // global step counter (device)
+  const float* drive_b;   // NULL: no input drive
+  const float* drive_u;   // [kDriveSteps] ring, u_t at t % kDriveSteps
+  float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
+};
+
+struct FinalizeArgs {
+  const double2* partials;
+  NetCtrl* ctrl;
+  int64_t* step;
+  float* r_hist;          // [kRHistSteps][n_nets]
+  int32_t n_nets;
+  int32_t num_blocks;
+  float dt;
+};
+
+// Launchers (leviathan_kernels.hip).  Each returns the launch status.
+hipError_t launch_step(int history, int layout, int ept, const StepArgs& a, hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
+hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, int32_t n_slots,
+                            const float* theta, int32_t n, hipStream_t s);
+hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t pitch, int32_t slot,
+                            float* out, int32_t n, hipStream_t s);
+
+}  // namespace lev

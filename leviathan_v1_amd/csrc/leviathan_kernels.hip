@@ -1,0 +1,404 @@
+// leviathan_kernels.hip -- gfx950 kernels of the fused LEVIATHAN step.
+//
+// One step of the reference (src/cuda/leviathan_kernel.cu:391-438) is four
+// launches: prep (:93-111), csr_physics (:114-167), integrate_and_reduce
+// (:170-234) and final_reduce_r (:237-264), plus a host-side gain update
+// (:430-435) behind a stream synchronisation.  Here one step is two launches:
+//
+//   step_kernel      prep + delayed gather + sin coupling + Hebbian update +
+//                    integrate + history write + FP64 partial sums of
+//                    cos/sin, for one row block per 256-lane workgroup;
+//   finalize_kernel  FP64 sum of the partials of each network, r, the FP64
+//                    gain controller, and the step counter.
+//
+// The fusion is legal because a row only writes its own state (theta,
+// theta_hat, omega_hat, its incoming edges' weights and its own history
+// slot), and the history ring has S = max_delay + 2 slots: the slot written
+// in step t, (t+1) % S, is never one a delay 0..max_delay reads in step t.
+// The reference needs the kernel boundary because its ring has max_delay+1
+// slots and the write slot aliases the max_delay read slot.
+//
+// Row blocks (built on the host) are either a run of up to 256 consecutive
+// rows holding at most 256*EPT incoming edges in total (stream mode: the
+// edges of the whole run are streamed with consecutive lanes on consecutive
+// edges, so column indices, delays and weights move in full cache lines
+// whatever the degrees), or a single row with more edges (long-row mode:
+// the whole workgroup strides over that row).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "leviathan_internal.h"
+#include "step_kernel.h"
+
+namespace lev {
+
+template <class H, int LAYOUT, int EPT>
+__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
+  constexpr int CAP = kBlock * EPT;
+  __shared__ float s_theta[kBlock];
+  __shared__ float s_gate[kBlock];
+  __shared__ uint8_t s_erow[CAP];
+  __shared__ float s_contrib[CAP];
+  __shared__ double s_red[2][kBlock / 64];
+
+  const int tid = threadIdx.x;
+  const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
+  const int4 blk = a.blocks[b];
+  const int32_t r0 = blk.x, r1 = blk.y, net = blk.z;
+  const int64_t step = *a.step;
+  const int32_t S = a.n_slots;
+  const int32_t cur = (int32_t)(step % S);
+  const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
+  const float g = (float)a.ctrl[net].g;
+  const float dt = a.dt;
+  const int32_t e0 = a.row_ptr[r0];
+  const int32_t e1 = a.row_ptr[r1];
+  const int32_t nnz = e1 - e0;
+  const int32_t nrows = r1 - r0;
+  const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
+  const int64_t pitch = a.ring_pitch;
+  float u_t = 0.0f;
+  if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
+
+  double my_c = 0.0, my_s = 0.0;
+
+  if (!(blk.w & kBlockLongRow)) {
+    // ---------------- stream mode ----------------
+    float th = 0.f, eps = 0.f, thh_new = 0.f, omh_new = 0.f;
+    int32_t rs = 0, re = 0;
+    if (tid < nrows) {
+      const int32_t i = r0 + tid;
+      th = a.theta[i];
+      const float thh = a.theta_hat[i];
+      const float omh = a.omega_hat[i];
+      // prep (:101-109): eps from the wrapped prediction error, then the
+      // predictor advances with the old omega_hat.
+      eps = sinf(wrap_pm_pi_div(th - thh));
+      thh_new = thh + dt * omh;
+      omh_new = omh + a.kappa_w * dt * eps;
+      s_theta[tid] = th;
+      s_gate[tid] = 1.0f - a.gamma * fabsf(eps);
+      rs = a.row_ptr[i] - e0;
+      re = a.row_ptr[i + 1] - e0;
+      for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
+    }
+    __syncthreads();
+
+    int32_t jj[EPT];
+    int32_t dd[EPT];
+    float ww[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      if (le < nnz) {
+        jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
+        dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+        ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
+      }
+    }
+    float hh[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      if (le < nnz) {
+        int32_t slot = cur - dd[k];
+        slot += (slot < 0) ? S : 0;
+        hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      if (le < nnz) {
+        const int lr = s_erow[le];
+        const float diff = wrap_pm_pi(hh[k] - s_theta[lr]);
+        float sn, cs;
+        sincosf(diff, &sn, &cs);
+        float wv = ww[k];
+        s_contrib[le] = wv * sn;  // coupling uses the pre-update weight (:154)
+        const float dw = dt * (a.eta * cs * s_gate[lr] - a.lam * wv);
+        wv += dw;
+        if (!__builtin_isfinite(wv)) wv = a.w_min;
+        wv = fmaxf(a.w_min, fminf(a.w_max, wv));
+        __builtin_nontemporal_store(wv, a.w + e0 + le);
+      }
+    }
+    __syncthreads();
+
+    if (tid < nrows) {
+      const int32_t i = r0 + tid;
+      CompSum acc;
+      for (int32_t q = rs; q < re; ++q) acc.add(s_contrib[q]);
+      float om = a.omega[i];
+      if (a.drive_b) om += a.drive_b[i] * u_t;
+      float x = th + dt * (om + g * acc.value() - a.alpha * eps);
+      x = wrap_0_2pi_div(x);
+      if (!__builtin_isfinite(x)) x = 0.0f;
+      a.theta[i] = x;
+      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      a.theta_hat[i] = thh_new;
+      a.omega_hat[i] = omh_new;
+      float sx, cx;
+      sincosf(x, &sx, &cx);
+      my_c = (double)cx;
+      my_s = (double)sx;
+    }
+  } else {
+    // ---------------- long-row mode: one row, whole workgroup ----------------
+    const int32_t i = r0;
+    float th = 0.f, eps = 0.f, thh_new = 0.f, omh_new = 0.f;
+    if (tid == 0) {
+      th = a.theta[i];
+      const float thh = a.theta_hat[i];
+      const float omh = a.omega_hat[i];
+      eps = sinf(wrap_pm_pi_div(th - thh));
+      thh_new = thh + dt * omh;
+      omh_new = omh + a.kappa_w * dt * eps;
+      s_theta[0] = th;
+      s_gate[0] = 1.0f - a.gamma * fabsf(eps);
+    }
+    __syncthreads();
+    const float th_i = s_theta[0];
+    const float gate = s_gate[0];
+    CompSum acc;
+    for (int32_t base = 0; base < nnz; base += kBlock * 4) {
+      int32_t jj[4], dd[4];
+      float ww[4], hh[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * kBlock + tid;
+        if (le < nnz) {
+          jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
+          dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+          ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * kBlock + tid;
+        if (le < nnz) {
+          int32_t slot = cur - dd[k];
+          slot += (slot < 0) ? S : 0;
+          hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * kBlock + tid;
+        if (le < nnz) {
+          const float diff = wrap_pm_pi(hh[k] - th_i);
+          float sn, cs;
+          sincosf(diff, &sn, &cs);
+          float wv = ww[k];
+          acc.add(wv * sn);
+          const float dw = dt * (a.eta * cs * gate - a.lam * wv);
+          wv += dw;
+          if (!__builtin_isfinite(wv)) wv = a.w_min;
+          wv = fmaxf(a.w_min, fminf(a.w_max, wv));
+          __builtin_nontemporal_store(wv, a.w + e0 + le);
+        }
+      }
+    }
+    // FP64 tree over the lanes' compensated partials (fixed order).
+    double part = wave_sum((double)acc.s + (double)acc.c);
+    if ((tid & 63) == 0) s_red[0][tid >> 6] = part;
+    __syncthreads();
+    if (tid == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlock / 64; ++q) tot += s_red[0][q];
+      float om = a.omega[i];
+      if (a.drive_b) om += a.drive_b[i] * u_t;
+      float x = th + dt * (om + g * (float)tot - a.alpha * eps);
+      x = wrap_0_2pi_div(x);
+      if (!__builtin_isfinite(x)) x = 0.0f;
+      a.theta[i] = x;
+      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      a.theta_hat[i] = thh_new;
+      a.omega_hat[i] = omh_new;
+      float sx, cx;
+      sincosf(x, &sx, &cx);
+      my_c = (double)cx;
+      my_s = (double)sx;
+    }
+    __syncthreads();  // s_red is reused below
+  }
+
+  // Block partial of sum cos / sum sin of the new phases, in FP64.
+  const double wc = wave_sum(my_c);
+  const double ws = wave_sum(my_s);
+  if ((tid & 63) == 0) {
+    s_red[0][tid >> 6] = wc;
+    s_red[1][tid >> 6] = ws;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double c = 0.0, s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      c += s_red[0][q];
+      s += s_red[1][q];
+    }
+    a.partials[b] = make_double2(c, s);
+  }
+}
+
+// r and the homeostatic gain controller (leviathan_kernel.cu:237-264 and
+// :430-435), in FP64, for every network; then the step counter advances.
+__global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs a) {
+  __shared__ double s_red[2][kFinalizeThreads / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t step = *a.step;
+  const int64_t hslot = step % kRHistSteps;
+
+  auto finish = [&](int net, double c, double s) {
+    NetCtrl& nc = a.ctrl[net];
+    const double r = sqrt(c * c + s * s) / (double)nc.n_nodes;
+    nc.r = r;
+    a.r_hist[hslot * a.n_nets + net] = (float)r;
+    if (nc.controller) {
+      double gn = nc.g + (double)a.dt * (nc.beta * (nc.r0 - r) - nc.mu * (nc.g - nc.g_rest));
+      gn = fmin(nc.g_max, gn);
+      gn = fmax(nc.g_min, gn);
+      nc.g = gn;
+    }
+  };
+
+  if (a.n_nets == 1) {
+    double c = 0.0, s = 0.0;
+    for (int32_t q = tid; q < a.num_blocks; q += kFinalizeThreads) {
+      const double2 p = a.partials[q];
+      c += p.x;
+      s += p.y;
+    }
+    c = wave_sum(c);
+    s = wave_sum(s);
+    if (lane == 0) {
+      s_red[0][wave] = c;
+      s_red[1][wave] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double tc = 0.0, ts = 0.0;
+      for (int q = 0; q < kFinalizeThreads / 64; ++q) {
+        tc += s_red[0][q];
+        ts += s_red[1][q];
+      }
+      finish(0, tc, ts);
+    }
+  } else {
+    for (int net = wave; net < a.n_nets; net += kFinalizeThreads / 64) {
+      const int32_t lo = a.ctrl[net].blk_begin, hi = a.ctrl[net].blk_end;
+      double c = 0.0, s = 0.0;
+      for (int32_t q = lo + lane; q < hi; q += 64) {
+        const double2 p = a.partials[q];
+        c += p.x;
+        s += p.y;
+      }
+      c = wave_sum(c);
+      s = wave_sum(s);
+      if (lane == 0) finish(net, c, s);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) *a.step = step + 1;
+}
+
+template <class H, int LAYOUT>
+__global__ void fill_ring_kernel(H* ring, int64_t pitch, int32_t n_slots, const float* theta, int32_t n) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)n * n_slots;
+  if (idx >= total) return;
+  int32_t node, slot;
+  if constexpr (LAYOUT == 0) {
+    node = (int32_t)(idx / n_slots);
+    slot = (int32_t)(idx % n_slots);
+  } else {
+    slot = (int32_t)(idx / n);
+    node = (int32_t)(idx % n);
+  }
+  ring[ring_index<LAYOUT>(pitch, node, slot)] = HistCodec<H>::encode(theta[node]);
+}
+
+template <class H, int LAYOUT>
+__global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, float* out, int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, i, slot)]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+template <class H, int LAYOUT>
+static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
+  const dim3 grid(a.num_blocks), block(kBlock);
+  switch (ept) {
+    case 4:
+      step_kernel<H, LAYOUT, 4><<<grid, block, 0, s>>>(a);
+      break;
+    case 8:
+      step_kernel<H, LAYOUT, 8><<<grid, block, 0, s>>>(a);
+      break;
+    case 16:
+      step_kernel<H, LAYOUT, 16><<<grid, block, 0, s>>>(a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_step(int history, int layout, int ept, const StepArgs& a, hipStream_t s) {
+  if (a.num_blocks <= 0) return hipSuccess;
+  if (history == 0) {
+    return layout == 0 ? launch_step_t<float, 0>(ept, a, s) : launch_step_t<float, 1>(ept, a, s);
+  }
+  return layout == 0 ? launch_step_t<uint16_t, 0>(ept, a, s) : launch_step_t<uint16_t, 1>(ept, a, s);
+}
+
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+  finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, int32_t n_slots,
+                            const float* theta, int32_t n, hipStream_t s) {
+  const int64_t total = (int64_t)n * n_slots;
+  if (total == 0) return hipSuccess;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  if (history == 0) {
+    if (layout == 0)
+      fill_ring_kernel<float, 0><<<grid, block, 0, s>>>((float*)ring, pitch, n_slots, theta, n);
+    else
+      fill_ring_kernel<float, 1><<<grid, block, 0, s>>>((float*)ring, pitch, n_slots, theta, n);
+  } else {
+    if (layout == 0)
+      fill_ring_kernel<uint16_t, 0><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, n_slots, theta, n);
+    else
+      fill_ring_kernel<uint16_t, 1><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, n_slots, theta, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t pitch, int32_t slot,
+                            float* out, int32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (history == 0) {
+    if (layout == 0)
+      read_slot_kernel<float, 0><<<grid, block, 0, s>>>((const float*)ring, pitch, slot, out, n);
+    else
+      read_slot_kernel<float, 1><<<grid, block, 0, s>>>((const float*)ring, pitch, slot, out, n);
+  } else {
+    if (layout == 0)
+      read_slot_kernel<uint16_t, 0><<<grid, block, 0, s>>>((const uint16_t*)ring, pitch, slot, out, n);
+    else
+      read_slot_kernel<uint16_t, 1><<<grid, block, 0, s>>>((const uint16_t*)ring, pitch, slot, out, n);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace lev

@@ -1,0 +1,240 @@
+"""``LeviathanEngine``: the Python surface of the gfx950 step library.
+
+Mirrors the reference's pybind11 class (src/bindings/leviathan_pybind11.cpp:27-104):
+same constructor arguments, same methods, same error behaviour (a
+``RuntimeError`` for a wrong-sized ``set_theta``), with the GIL released in
+every call (ctypes does this for CDLL functions).  Arrays are cast to the
+expected dtype and made contiguous like ``py::array_t<T>``'s forcecast, and,
+unlike the reference, their lengths are checked.
+
+Extensions: keyword options (history format and layout, physics parameters,
+ensembles), ``step_n``, ``get_weights``, ``get_predictor``, ``get_history``,
+``profile_steps`` and an input drive for reservoir use.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+
+_PARAM_NAMES = ("eta", "lam", "gamma", "alpha_eps", "kappa_w", "w_min", "w_max", "g0", "r0", "beta", "mu",
+                "g_rest", "g_min", "g_max")
+
+
+def _arr(a, dtype, n: Optional[int], name: str):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.ndim != 1:
+        raise ValueError(f"{name} must be one-dimensional")
+    if n is not None and a.shape[0] != n:
+        raise ValueError(f"{name} has {a.shape[0]} entries, expected {n}")
+    return a
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class LeviathanEngine:
+    """GPU-resident delayed Kuramoto network with adaptive coupling.
+
+    ``LeviathanEngine(N, max_delay, row_ptr, col_idx, delays, weights, theta,
+    theta_hat, omega)`` -- exactly the reference constructor
+    (leviathan_pybind11.cpp:32-42, :86-89).
+
+    Keyword extensions:
+      history: "f32" (default) or "u16" (exact-wrap uint16 angles)
+      layout:  "node" (ring[node][slot], default) or "slot" (ring[slot][node])
+      edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
+      graph_chunk: steps per captured HIP graph in step_n (0 disables graphs)
+      params: dict of physics parameters (names of LeviathanConfig)
+      n_nets, net_offsets, g0, controller: ensemble of independent networks
+    """
+
+    def __init__(self, N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega, *,
+                 history: str = "f32", layout: str = "node", edges_per_thread: int = 8, graph_chunk: int = 100,
+                 params: Optional[dict] = None, n_nets: int = 1, net_offsets: Optional[Sequence[int]] = None,
+                 g0: Optional[Sequence[float]] = None, controller: Optional[Sequence[bool]] = None):
+        self._h = None
+        lib = _lib.lib()
+        N = int(N)
+        if N <= 0:
+            raise ValueError("N must be positive")
+        row_ptr = _arr(row_ptr, np.int32, N + 1, "row_ptr")
+        E = int(row_ptr[-1])
+        col_idx = _arr(col_idx, np.int32, E, "col_idx")
+        delays = _arr(delays, np.uint8, E, "delays")
+        weights = _arr(weights, np.float32, E, "weights")
+        theta = _arr(theta, np.float32, N, "theta")
+        theta_hat = _arr(theta_hat, np.float32, N, "theta_hat")
+        omega = _arr(omega, np.float32, N, "omega")
+        self.N, self.E, self.max_delay = N, E, int(max_delay)
+        self.n_nets = int(n_nets)
+        extended = (history != "f32" or layout != "node" or edges_per_thread != 8 or graph_chunk != 100
+                    or params or n_nets != 1 or net_offsets is not None or g0 is not None
+                    or controller is not None)
+        if not extended:
+            h = lib.leviathan_init(N, self.max_delay, _p(row_ptr), _p(col_idx), _p(delays), _p(weights),
+                                   _p(theta), _p(theta_hat), _p(omega))
+        else:
+            cfg = _lib.default_config()
+            if history not in _lib.HISTORY:
+                raise ValueError(f"history must be one of {list(_lib.HISTORY)}")
+            if layout not in _lib.LAYOUT:
+                raise ValueError(f"layout must be one of {list(_lib.LAYOUT)}")
+            cfg.history = _lib.HISTORY[history]
+            cfg.layout = _lib.LAYOUT[layout]
+            cfg.edges_per_thread = int(edges_per_thread)
+            cfg.graph_chunk = int(graph_chunk)
+            for k, v in (params or {}).items():
+                if k == "gain_controller":
+                    cfg.gain_controller = int(bool(v))
+                elif k in _PARAM_NAMES:
+                    setattr(cfg, k, float(v))
+                else:
+                    raise ValueError(f"unknown parameter {k!r}")
+            offs = None if net_offsets is None else _arr(net_offsets, np.int32, self.n_nets + 1, "net_offsets")
+            g0a = None if g0 is None else _arr(g0, np.float32, self.n_nets, "g0")
+            ctl = None if controller is None else _arr(np.asarray(controller, dtype=bool).astype(np.int32),
+                                                       np.int32, self.n_nets, "controller")
+            h = lib.leviathan_init_ex(ctypes.byref(cfg), N, self.max_delay, _p(row_ptr), _p(col_idx), _p(delays),
+                                      _p(weights), _p(theta), _p(theta_hat), _p(omega), self.n_nets, _p(offs),
+                                      _p(g0a), _p(ctl))
+        if not h:
+            raise RuntimeError("leviathan_init failed: " + _lib.last_error())
+        self._h = ctypes.c_void_p(h)
+        self.history = history
+        self.layout = layout
+
+    # ------------------------------------------------------------------ reference surface
+    def step(self, dt: float) -> float:
+        """One step; returns r (leviathan_pybind11.cpp:44-47)."""
+        r = _lib.lib().leviathan_step(self._h, float(dt))
+        if r != r:  # NaN signals failure
+            _lib.check("step")
+        return float(r)
+
+    def get_theta(self) -> np.ndarray:
+        """Copy of the phases as a new float32 array (leviathan_pybind11.cpp:50-55)."""
+        out = np.empty(self.N, dtype=np.float32)
+        _lib.lib().leviathan_get_theta(self._h, _p(out))
+        _lib.check("get_theta")
+        return out
+
+    def set_theta(self, theta) -> None:
+        """Upload phases and reset the history (leviathan_pybind11.cpp:57-65)."""
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        if theta.size != self.N:
+            raise RuntimeError(f"theta array size mismatch: expected {self.N}, got {theta.size}")
+        _lib.lib().leviathan_set_theta(self._h, _p(theta.reshape(-1)))
+        _lib.check("set_theta")
+
+    def set_gain(self, g: float) -> None:
+        _lib.lib().leviathan_set_gain(self._h, float(g))
+        _lib.check("set_gain")
+
+    def get_gain(self) -> float:
+        g = _lib.lib().leviathan_get_gain(self._h)
+        _lib.check("get_gain")
+        return float(g)
+
+    def set_gain_controller(self, enabled: bool) -> None:
+        _lib.lib().leviathan_set_gain_controller(self._h, bool(enabled))
+        _lib.check("set_gain_controller")
+
+    def get_vram_usage(self) -> int:
+        return int(_lib.lib().leviathan_get_vram_usage(self._h))
+
+    def reset_weights(self, weights) -> None:
+        weights = _arr(weights, np.float32, self.E, "weights")
+        _lib.lib().leviathan_reset_weights(self._h, _p(weights))
+        _lib.check("reset_weights")
+
+    # ------------------------------------------------------------------ extensions
+    def step_n(self, dt: float, n: int) -> np.ndarray:
+        """n steps with one synchronisation; returns r as float32[n] (or [n, n_nets])."""
+        n = int(n)
+        out = np.empty((n, self.n_nets), dtype=np.float32)
+        if _lib.lib().leviathan_step_n(self._h, float(dt), n, _p(out)) != 0:
+            _lib.check("step_n")
+        return out[:, 0] if self.n_nets == 1 else out
+
+    def step_n_driven(self, dt: float, u) -> np.ndarray:
+        u = _arr(u, np.float32, None, "u")
+        n = u.shape[0]
+        out = np.empty((n, self.n_nets), dtype=np.float32)
+        if _lib.lib().leviathan_step_n_driven(self._h, float(dt), n, _p(u), _p(out)) != 0:
+            _lib.check("step_n_driven")
+        return out[:, 0] if self.n_nets == 1 else out
+
+    def set_drive(self, b) -> None:
+        """omega_i(t) = omega_i + b_i * u_t (None switches the drive off)."""
+        arr = None if b is None else _arr(b, np.float32, self.N, "drive_b")
+        if _lib.lib().leviathan_set_drive(self._h, _p(arr)) != 0:
+            _lib.check("set_drive")
+
+    def get_weights(self) -> np.ndarray:
+        out = np.empty(self.E, dtype=np.float32)
+        if _lib.lib().leviathan_get_weights(self._h, _p(out)) != 0:
+            _lib.check("get_weights")
+        return out
+
+    def get_predictor(self):
+        th = np.empty(self.N, dtype=np.float32)
+        om = np.empty(self.N, dtype=np.float32)
+        if _lib.lib().leviathan_get_predictor(self._h, _p(th), _p(om)) != 0:
+            _lib.check("get_predictor")
+        return th, om
+
+    def get_history(self, delay: int) -> np.ndarray:
+        out = np.empty(self.N, dtype=np.float32)
+        if _lib.lib().leviathan_get_history(self._h, int(delay), _p(out)) != 0:
+            _lib.check("get_history")
+        return out
+
+    def set_gain_net(self, net: int, g: float) -> None:
+        if _lib.lib().leviathan_set_gain_net(self._h, int(net), float(g)) != 0:
+            _lib.check("set_gain_net")
+
+    def get_gain_net(self, net: int) -> float:
+        g = _lib.lib().leviathan_get_gain_net(self._h, int(net))
+        _lib.check("get_gain_net")
+        return float(g)
+
+    def set_gain_controller_net(self, net: int, enabled: bool) -> None:
+        if _lib.lib().leviathan_set_gain_controller_net(self._h, int(net), bool(enabled)) != 0:
+            _lib.check("set_gain_controller_net")
+
+    @property
+    def step_count(self) -> int:
+        return int(_lib.lib().leviathan_step_count(self._h))
+
+    @property
+    def num_blocks(self) -> int:
+        return int(_lib.lib().leviathan_num_blocks(self._h))
+
+    def profile_steps(self, dt: float, n: int) -> dict:
+        """Times n eager steps with HIP events on the engine's stream."""
+        out = np.zeros(3, dtype=np.float64)
+        if _lib.lib().leviathan_profile_steps(self._h, float(dt), int(n), _p(out)) != 0:
+            _lib.check("profile_steps")
+        return {"step_kernel_ms": float(out[0]), "finalize_kernel_ms": float(out[1]), "wall_ms": float(out[2])}
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().leviathan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,65 @@
+/* leviathan_oracle.c -- CPU restatement of LEVIATHAN's discrete step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product path links or calls this
+ * file: it is the checker for the gfx950 kernels (tests/, smoke(), and the
+ * cpu_baseline leg of bench.py) and the FP32 CPU baseline.
+ *
+ * Two instantiations of one body (oracle_step.inc):
+ *   lev_oracle_step_f64  -- FP64 arithmetic, single thread, sequential edge
+ *                           sums: the correctness oracle.
+ *   lev_oracle_step_f32  -- FP32 arithmetic, OpenMP over rows, mirroring the
+ *                           reference's four-kernel FP32 step: the CPU
+ *                           baseline ("port") timed beside the GPU.
+ *
+ * Both follow the reference's kernel order exactly (src/cuda/leviathan_kernel.cu):
+ *   prep_kernel                  :93-111   eps, theta_hat, omega_hat
+ *   csr_physics_kernel           :114-167  delayed gather, coupling, Hebbian
+ *   integrate_and_reduce_kernel  :170-234  phase update, history write, sums
+ *   final_reduce_r_kernel        :237-264  r = |sum e^{i theta}| / N
+ *   leviathan_step               :391-438  buf_idx advance, host gain control
+ * with the history ring of B = max_delay + 1 node-major slots (:277, :327) and
+ * the parameter defaults of :281-297 (supplied by the caller).
+ *
+ * Extension beyond the reference (no counterpart there): an optional input
+ * drive omega_i(t) = omega_i + drive_b[i] * u_t, used by the reservoir readout.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+
+typedef struct {
+  double eta, lam, gamma, alpha_eps, kappa_w;
+  double w_min, w_max;
+  double r0, beta, mu, g_rest, g_min, g_max;
+  int32_t gain_controller;
+} lev_oracle_params;
+
+/* FP64 instantiation */
+#define REAL double
+#define SUFFIX f64
+#define SIN sin
+#define COS cos
+#define FLOOR floor
+#define SQRT sqrt
+#define TWO_PI_R 6.283185307179586476925286766559
+#define USE_OMP 0
+#include "oracle_step.inc"
+#undef REAL
+#undef SUFFIX
+#undef SIN
+#undef COS
+#undef FLOOR
+#undef SQRT
+#undef TWO_PI_R
+#undef USE_OMP
+
+/* FP32 instantiation */
+#define REAL float
+#define SUFFIX f32
+#define SIN sinf
+#define COS cosf
+#define FLOOR floorf
+#define SQRT sqrtf
+#define TWO_PI_R 6.283185307179586476925286766559f
+#define USE_OMP 1
+#include "oracle_step.inc"
