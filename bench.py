@@ -60,11 +60,17 @@ def make_workload(name, n_override=None):
     raise ValueError(name)
 
 
-def run_form(wl, history, steps, warmup, layout="node", ept=8, profile_steps=20):
+def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", profile_steps=20):
     from leviathan_v1_amd import LeviathanEngine
 
-    eng = LeviathanEngine(*wl.engine_args(), history=history, layout=layout, edges_per_thread=ept)
+    t0 = time.perf_counter()
+    eng = LeviathanEngine(*wl.engine_args(), history=history, layout=layout, edges_per_thread=ept,
+                          schedule=schedule)
+    log(f"{history}: engine ready in {time.perf_counter() - t0:.1f} s")
     try:
+        tuned = eng.autotune(wl.dt, steps=5) if schedule == "auto" else None
+        chosen = eng.get_schedule()
+        log(f"{history}: autotune {tuned} -> {chosen}")
         eng.step_n(wl.dt, warmup)
         t0 = time.perf_counter()
         r = eng.step_n(wl.dt, steps)  # returns after the stream is synchronised
@@ -73,11 +79,13 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, profile_steps=20)
         theta = eng.get_theta()
         ok = bool(np.isfinite(theta).all() and np.isfinite(r).all() and 0 <= r.min() and r.max() <= 1)
         return {
-            "history": history, "layout": layout, "edges_per_thread": ept,
+            "history": history, "layout": layout, "schedule": chosen[0], "edges_per_thread": chosen[1],
+            "autotune_ms_per_step": tuned,
             "seconds": el, "ms_per_step": 1e3 * el / steps,
             "edge_steps_per_s": wl.E * steps / el,
             "hbm_gbs": algorithmic_bytes(wl.N, wl.E, history) * steps / el / 1e9,
             "step_kernel_ms": prof["step_kernel_ms"], "finalize_kernel_ms": prof["finalize_kernel_ms"],
+            "scatter_kernel_ms": prof["scatter_kernel_ms"],
             "r_last": float(r[-1]), "invariants_ok": ok, "vram_bytes": eng.get_vram_usage(),
             "row_blocks": eng.num_blocks,
         }
@@ -103,6 +111,28 @@ def cpu_baseline(wl, budget_s=15.0, max_steps=50):
                       f"leviathan_kernel.cu:93-264,391-438), {el:.1f} s"}
 
 
+def copy_calibration(nbytes: int, reps: int = 3) -> None:
+    """A plain device-to-device copy of known size (HIP runtime blit kernel), so that
+    profiler byte counters can be scaled: counter(copy) / (2 * nbytes * reps)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b),
+                                                                                  ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    log(f"copy calibration: {reps} x {nbytes} bytes device-to-device")
+
+
 def load_traffic(workload, history):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -123,8 +153,11 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
     ap.add_argument("--layout", default="node", choices=["node", "slot"])
     ap.add_argument("--ept", type=int, default=8)
+    ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="f32,u16")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run a plain device copy of BYTES (profiler counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU "
@@ -136,17 +169,22 @@ def main():
         print(json.dumps(ensemble_bench.run(args.steps, args.warmup)))
         return
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     t0 = time.perf_counter()
     wl = make_workload(args.workload, args.n)
     log(f"workload {wl.name}: N={wl.N} E={wl.E} built in {time.perf_counter() - t0:.1f} s")
     forms = {}
     for h in args.forms.split(","):
-        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept)
+        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept, args.schedule)
         log(h, json.dumps(forms[h]))
     head = forms.get("u16") or forms["f32"]
     hist = head["history"]
     alg = algorithmic_bytes(wl.N, wl.E, hist)
-    achieved = alg / (head["step_kernel_ms"] * 1e-3) / 1e9
+    # the step's kernels: the fused step kernel (direct) or scatter + bucket kernels (binned)
+    step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"]
+    kernels = "step_kernel" if head["schedule"] == "direct" else "scatter_kernel+bucket_kernel"
+    achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
         log("cpu", json.dumps(cpu))
@@ -172,7 +210,7 @@ def main():
                                 if "u16" in forms and "f32" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload, hist),
-                     "kernel": "step_kernel", "algorithmic_bytes_per_launch": alg},
+                     "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

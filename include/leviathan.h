@@ -101,10 +101,19 @@ typedef struct {
   int32_t layout;          /* LEV_LAYOUT_* */
   int32_t edges_per_thread;/* 4, 8 or 16: a 256-lane block streams up to 256*this edges */
   int32_t graph_chunk;     /* steps per captured HIP graph in leviathan_step_n (0: no graphs) */
+  int32_t schedule;        /* LEV_SCHEDULE_* */
+  int32_t bucket_edges;    /* binned schedule: max incoming edges per bucket (0: automatic) */
 } LeviathanConfig;
 
+enum {
+  LEV_SCHEDULE_DIRECT = 0, /* one fused kernel; delayed phases gathered per edge */
+  LEV_SCHEDULE_BINNED = 1, /* two phases: source-ordered scatter into destination buckets,
+                              then the fused row update from LDS (graphs without locality) */
+  LEV_SCHEDULE_AUTO = 2    /* both plans are built; leviathan_autotune picks */
+};
+
 /* Fills cfg with the defaults above (FP32 node-major history, 8 edges per
- * thread, 100-step graph chunks). */
+ * thread, 100-step graph chunks, direct schedule). */
 void leviathan_config_default(LeviathanConfig* cfg);
 
 /* Ensemble constructor: n_nets independent networks stored as one
@@ -147,11 +156,26 @@ int64_t leviathan_step_count(LeviathanHandle* handle);
 int leviathan_num_blocks(LeviathanHandle* handle);
 
 /* Runs n eager steps and times them with HIP events on the handle's stream.
- * out[0] = mean duration of the fused step kernel (ms per launch),
+ * out[0] = mean duration of the step's main kernel (ms per launch): the fused
+ *          step kernel (direct schedule) or the bucket kernel (binned),
  * out[1] = mean duration of the per-step finalize kernel (ms per launch),
- * out[2] = wall time of the n steps between the first and last event (ms).
+ * out[2] = wall time of the n steps between the first and last event (ms),
+ * out[3] = mean duration of the scatter kernel (binned schedule; 0 otherwise).
  * Returns 0 or -1. */
 int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* out);
+
+/* Schedule control (handles created with LEV_SCHEDULE_BINNED or _AUTO hold
+ * both plans).  leviathan_set_schedule switches between them;
+ * leviathan_autotune times `steps` steps of every candidate (direct with 4, 8
+ * and 16 edges per lane, binned) from the same state, restores the state,
+ * keeps the fastest and writes the mean milliseconds per step of each
+ * candidate to ms_out[4] (NaN for a candidate not available).  The
+ * schedules give the same phases and weights (the per-row arithmetic and
+ * summation order are shared); r differs only in the grouping of its FP64
+ * partial sums.  Return 0 or -1. */
+int leviathan_set_schedule(LeviathanHandle* handle, int schedule, int edges_per_thread);
+int leviathan_get_schedule(LeviathanHandle* handle, int* schedule, int* edges_per_thread);
+int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_out);
 
 /* Input drive (reservoir): omega_i(t) = omega_i + drive_b[i] * u_t.
  * leviathan_set_drive uploads drive_b[N] (NULL switches the drive off);

@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 GRAPH_SO = os.path.join(PKG, "libleviathan_graph.so")
 HIP_SO = os.path.join(PKG, "libleviathan_hip.so")
 
-HIP_SOURCES = ["leviathan_api.hip", "leviathan_kernels.hip", "reservoir.hip"]
+HIP_SOURCES = ["leviathan_api.hip", "leviathan_kernels.hip", "leviathan_binned.hip", "plan.cpp", "reservoir.hip"]
 HIP_HEADERS = ["leviathan_internal.h", "step_kernel.h"]
 
 # Packed FP32 instructions are switched off for every device compile.
@@ -34,7 +34,7 @@ HIPCC_FLAGS = [
     "-shared",
     "-fno-gpu-rdc",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast",
+    "-ffp-contract=off",  # explicit fmaf only: identical rounding in every kernel
     "-Wall",
     "-Wno-unused-command-line-argument",
 ]

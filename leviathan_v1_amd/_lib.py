@@ -35,11 +35,13 @@ class LeviathanConfig(ctypes.Structure):
         ("gain_controller", c_int32),
         ("history", c_int32), ("layout", c_int32), ("edges_per_thread", c_int32),
         ("graph_chunk", c_int32),
+        ("schedule", c_int32), ("bucket_edges", c_int32),
     ]
 
 
 HISTORY = {"f32": 0, "u16": 1}
 LAYOUT = {"node": 0, "slot": 1}
+SCHEDULE = {"direct": 0, "binned": 1, "auto": 2}
 
 # name -> (restype, argtypes); every symbol include/leviathan.h declares.
 SIGNATURES = {
@@ -71,6 +73,9 @@ SIGNATURES = {
     "leviathan_num_blocks": (c_int, [vp]),
     "leviathan_profile_steps": (c_int, [vp, c_float, c_int, vp]),
     "leviathan_set_drive": (c_int, [vp, vp]),
+    "leviathan_set_schedule": (c_int, [vp, c_int, c_int]),
+    "leviathan_get_schedule": (c_int, [vp, vp, vp]),
+    "leviathan_autotune": (c_int, [vp, c_float, c_int, vp]),
     "leviathan_step_n_driven": (c_int, [vp, c_float, c_int, vp, vp]),
 }
 

@@ -16,6 +16,7 @@
 
 #include "leviathan.h"
 #include "leviathan_internal.h"
+#include "plan.h"
 
 using lev::FinalizeArgs;
 using lev::NetCtrl;
@@ -49,6 +50,26 @@ struct GraphEntry {
   hipGraphExec_t exec;
 };
 
+// One schedule's work decomposition on the device.
+constexpr int kPlanDirect4 = 0, kPlanDirect8 = 1, kPlanDirect16 = 2, kPlanBinned = 3, kNumPlans = 4;
+
+struct DevPlan {
+  bool built = false;
+  int kind = LEV_SCHEDULE_DIRECT;
+  int ept = 8;
+  int4* d_items = nullptr;
+  int32_t n_items = 0;
+  int2* d_ranges = nullptr;
+  // binned only
+  int4* d_sources = nullptr;
+  int32_t n_sources = 0;
+  int2* d_sb_index = nullptr;
+  int32_t* d_sb_bounds = nullptr;
+  size_t lds = 0;
+};
+
+int plan_of_ept(int ept) { return ept == 4 ? kPlanDirect4 : ept == 16 ? kPlanDirect16 : kPlanDirect8; }
+
 }  // namespace
 
 struct LeviathanHandle {
@@ -60,7 +81,6 @@ struct LeviathanHandle {
   int32_t n_slots = 0;      // S = max_delay + 2
   int64_t ring_pitch = 0;
   int32_t n_nets = 1;
-  int32_t num_blocks = 0;
   int64_t step = 0;         // host mirror of the device step counter
   size_t vram = 0;
 
@@ -73,8 +93,16 @@ struct LeviathanHandle {
   float* d_omega = nullptr;
   float* d_omega_hat = nullptr;
   void* d_ring = nullptr;
-  int4* d_blocks = nullptr;
   double2* d_partials = nullptr;
+  int32_t n_partials = 0;
+  DevPlan plans[kNumPlans];
+  int active = kPlanDirect8;
+  // binned schedule arrays
+  int32_t* d_csc_ptr = nullptr;
+  uint8_t* d_delay_csc = nullptr;
+  uint32_t* d_pos = nullptr;
+  void* d_hbuf = nullptr;
+  uint16_t* d_perm = nullptr;
   NetCtrl* d_ctrl = nullptr;
   int64_t* d_step = nullptr;
   float* d_r_hist = nullptr;
@@ -114,11 +142,17 @@ void destroy(LeviathanHandle* h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graphs(h);
-  void* ptrs[] = {h->d_row_ptr, h->d_col,   h->d_delay,  h->d_w,      h->d_theta,   h->d_theta_hat,
-                  h->d_omega,   h->d_omega_hat, h->d_ring, h->d_blocks, h->d_partials, h->d_ctrl,
-                  h->d_step,    h->d_r_hist, h->d_scratch, h->d_drive_b, h->d_drive_u};
+  void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
+                  h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
+                  h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
+                  h->d_pos,     h->d_hbuf,      h->d_perm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto& pl : h->plans) {
+    void* pp[] = {pl.d_items, pl.d_ranges, pl.d_sources, pl.d_sb_index, pl.d_sb_bounds};
+    for (void* p : pp)
+      if (p) (void)hipFree(p);
+  }
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -137,49 +171,7 @@ bool check_handle(LeviathanHandle* h) {
   return true;
 }
 
-// Row blocks: runs of <= 256 rows with <= cap edges, or one long row.
-// Blocks of a network are contiguous; within a network long rows come first
-// (heaviest first) so they start early.
-bool build_blocks(const int* row_ptr, int n_nets, const int* net_offsets, int cap,
-                  std::vector<int4>& blocks, std::vector<NetCtrl>& ctrl) {
-  blocks.clear();
-  for (int net = 0; net < n_nets; ++net) {
-    const int lo = net_offsets[net], hi = net_offsets[net + 1];
-    std::vector<int4> longs, shorts;
-    int r = lo;
-    while (r < hi) {
-      const int deg = row_ptr[r + 1] - row_ptr[r];
-      if (deg > cap) {
-        longs.push_back(make_int4(r, r + 1, net, lev::kBlockLongRow));
-        ++r;
-        continue;
-      }
-      const int start = r;
-      int nnz = 0;
-      while (r < hi && r - start < lev::kBlock) {
-        const int d = row_ptr[r + 1] - row_ptr[r];
-        if (d > cap || nnz + d > cap) break;
-        nnz += d;
-        ++r;
-      }
-      shorts.push_back(make_int4(start, r, net, 0));
-    }
-    std::stable_sort(longs.begin(), longs.end(), [&](const int4& a, const int4& b) {
-      return (row_ptr[a.x + 1] - row_ptr[a.x]) > (row_ptr[b.x + 1] - row_ptr[b.x]);
-    });
-    ctrl[net].blk_begin = (int32_t)blocks.size();
-    blocks.insert(blocks.end(), longs.begin(), longs.end());
-    blocks.insert(blocks.end(), shorts.begin(), shorts.end());
-    ctrl[net].blk_end = (int32_t)blocks.size();
-  }
-  if (blocks.size() > (size_t)std::numeric_limits<int32_t>::max()) {
-    set_err("too many row blocks");
-    return false;
-  }
-  return true;
-}
-
-StepArgs step_args(LeviathanHandle* h, float dt) {
+StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   StepArgs a{};
   a.row_ptr = h->d_row_ptr;
   a.col = h->d_col;
@@ -192,9 +184,9 @@ StepArgs step_args(LeviathanHandle* h, float dt) {
   a.ring = h->d_ring;
   a.ring_pitch = h->ring_pitch;
   a.n_slots = h->n_slots;
-  a.num_blocks = h->num_blocks;
+  a.num_blocks = pl.n_items;
   a.xcd_remap = 1;
-  a.blocks = h->d_blocks;
+  a.blocks = pl.d_items;
   a.partials = h->d_partials;
   a.ctrl = h->d_ctrl;
   a.step = h->d_step;
@@ -211,23 +203,59 @@ StepArgs step_args(LeviathanHandle* h, float dt) {
   return a;
 }
 
-FinalizeArgs fin_args(LeviathanHandle* h, float dt) {
+FinalizeArgs fin_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   FinalizeArgs f{};
   f.partials = h->d_partials;
   f.ctrl = h->d_ctrl;
   f.step = h->d_step;
   f.r_hist = h->d_r_hist;
+  f.ranges = pl.d_ranges;
   f.n_nets = h->n_nets;
-  f.num_blocks = h->num_blocks;
+  f.num_blocks = pl.n_items;
   f.dt = dt;
   return f;
 }
 
-// Enqueue one step (two launches) on the handle's stream.
+lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
+  lev::ScatterArgs sa{};
+  sa.csc_ptr = h->d_csc_ptr;
+  sa.delay = h->d_delay_csc;
+  sa.pos = h->d_pos;
+  sa.ring = h->d_ring;
+  sa.ring_pitch = h->ring_pitch;
+  sa.n_slots = h->n_slots;
+  sa.num_blocks = pl.n_sources;
+  sa.xcd_remap = 1;
+  sa.blocks = pl.d_sources;
+  sa.hbuf = h->d_hbuf;
+  sa.step = h->d_step;
+  return sa;
+}
+
+lev::BucketArgs bucket_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
+  lev::BucketArgs ba{};
+  ba.s = step_args(h, dt, pl);
+  ba.hbuf = h->d_hbuf;
+  ba.perm = h->d_perm;
+  ba.sb_index = pl.d_sb_index;
+  ba.sb_bounds = pl.d_sb_bounds;
+  return ba;
+}
+
+// Enqueue the step kernel(s) of the active plan, without the finalize.
+hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr) {
+  const DevPlan& pl = h->plans[h->active];
+  if (pl.kind == LEV_SCHEDULE_DIRECT) return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, step_args(h, dt, pl), h->stream);
+  hipError_t e = lev::launch_scatter(h->cfg.history, h->cfg.layout, scatter_args(h, pl), h->stream);
+  if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
+  if (e == hipSuccess) e = lev::launch_bucket(h->cfg.history, h->cfg.layout, bucket_args(h, dt, pl), pl.lds, h->stream);
+  return e;
+}
+
+// Enqueue one step on the handle's stream.
 bool enqueue_step(LeviathanHandle* h, float dt) {
-  const StepArgs a = step_args(h, dt);
-  LEV_HIP(lev::launch_step(h->cfg.history, h->cfg.layout, h->cfg.edges_per_thread, a, h->stream));
-  LEV_HIP(lev::launch_finalize(fin_args(h, dt), h->stream));
+  LEV_HIP(enqueue_main(h, dt));
+  LEV_HIP(lev::launch_finalize(fin_args(h, dt, h->plans[h->active]), h->stream));
   return true;
 }
 
@@ -380,6 +408,19 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     set_err("graph_chunk must be in [0, %d]", lev::kRHistSteps);
     return nullptr;
   }
+  if (cfg.schedule != LEV_SCHEDULE_DIRECT && cfg.schedule != LEV_SCHEDULE_BINNED && cfg.schedule != LEV_SCHEDULE_AUTO) {
+    set_err("unknown schedule %d", cfg.schedule);
+    return nullptr;
+  }
+  const size_t hist_bytes = cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+  if (cfg.bucket_edges == 0) cfg.bucket_edges = (int)(32768 / hist_bytes);
+  if (cfg.schedule != LEV_SCHEDULE_DIRECT &&
+      (cfg.bucket_edges < lev::kBlock * lev::kBucketSubEpt || cfg.bucket_edges > 65536 ||
+       (size_t)cfg.bucket_edges * hist_bytes > (size_t)lev::kMaxBucketLds)) {
+    set_err("bucket_edges must be in [%d, %d] for this history format", lev::kBlock * lev::kBucketSubEpt,
+            (int)std::min<size_t>(65536, lev::kMaxBucketLds / hist_bytes));
+    return nullptr;
+  }
   if (!validate_inputs(N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega)) return nullptr;
   std::vector<int> offs;
   if (n_nets <= 0) {
@@ -446,11 +487,39 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   const size_t ring_elems = cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)N * h->ring_pitch
                                                                : (size_t)h->n_slots * h->ring_pitch;
 
-  std::vector<int4> blocks;
+  // work decomposition: the configured direct plan always, the other direct
+  // plans and the binned plan when a schedule other than DIRECT is asked for
   h->h_ctrl.assign(n_nets, NetCtrl{});
-  if (!build_blocks(row_ptr, n_nets, offs.data(), lev::kBlock * cfg.edges_per_thread, blocks, h->h_ctrl))
-    return fail();
-  h->num_blocks = (int32_t)blocks.size();
+  std::string perr;
+  lev::HostPlan hp[3];
+  const int epts[3] = {4, 8, 16};
+  for (int q = 0; q < 3; ++q) {
+    if (cfg.schedule == LEV_SCHEDULE_DIRECT && epts[q] != cfg.edges_per_thread) continue;
+    if (!lev::build_direct_plan(row_ptr, n_nets, offs.data(), lev::kBlock * epts[q], hp[q], perr)) {
+      set_err("%s", perr.c_str());
+      return fail();
+    }
+    h->plans[q].built = true;
+    h->plans[q].kind = LEV_SCHEDULE_DIRECT;
+    h->plans[q].ept = epts[q];
+    h->plans[q].n_items = (int32_t)hp[q].items.size();
+  }
+  lev::HostBinned hb;
+  if (cfg.schedule != LEV_SCHEDULE_DIRECT) {
+    if (!lev::build_binned_plan(N, row_ptr, col_idx, delays, n_nets, offs.data(), cfg.bucket_edges, hb, perr)) {
+      set_err("%s", perr.c_str());
+      return fail();
+    }
+    DevPlan& pb = h->plans[kPlanBinned];
+    pb.built = true;
+    pb.kind = LEV_SCHEDULE_BINNED;
+    pb.n_items = (int32_t)hb.buckets.items.size();
+    pb.n_sources = (int32_t)hb.sources.size();
+    pb.lds = (size_t)std::max(hb.max_bucket_edges, 1) * hist_bytes;
+  }
+  h->active = cfg.schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : plan_of_ept(cfg.edges_per_thread);
+  for (const auto& pl : h->plans)
+    if (pl.built) h->n_partials = std::max(h->n_partials, pl.n_items);
   for (int k = 0; k < n_nets; ++k) {
     NetCtrl& c = h->h_ctrl[k];
     c.g = g0 ? (double)g0[k] : (double)cfg.g0;
@@ -469,8 +538,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       !dalloc(h, &h->d_delay, (size_t)h->E) || !dalloc(h, &h->d_w, (size_t)h->E) ||
       !dalloc(h, &h->d_theta, (size_t)N) || !dalloc(h, &h->d_theta_hat, (size_t)N) ||
       !dalloc(h, &h->d_omega, (size_t)N) || !dalloc(h, &h->d_omega_hat, (size_t)N) ||
-      !dalloc(h, (char**)&h->d_ring, ring_elems * hsize) || !dalloc(h, &h->d_blocks, blocks.size()) ||
-      !dalloc(h, &h->d_partials, blocks.size()) || !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
+      !dalloc(h, (char**)&h->d_ring, ring_elems * hsize) || !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
+      !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
       !dalloc(h, &h->d_step, 1) || !dalloc(h, &h->d_r_hist, (size_t)lev::kRHistSteps * n_nets) ||
       !dalloc(h, &h->d_scratch, (size_t)N) || !dalloc(h, &h->d_drive_u, (size_t)lev::kDriveSteps))
     return fail();
@@ -490,13 +559,44 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
             hip_ok(hipMemcpy(h->d_omega, omega, sizeof(float) * N, H2D), "copy omega") &&
             // omega_hat starts as omega (leviathan_kernel.cu:350)
             hip_ok(hipMemcpy(h->d_omega_hat, omega, sizeof(float) * N, H2D), "copy omega_hat") &&
-            hip_ok(hipMemcpy(h->d_blocks, blocks.data(), sizeof(int4) * blocks.size(), H2D), "copy blocks") &&
             hip_ok(hipMemcpy(h->d_ctrl, h->h_ctrl.data(), sizeof(NetCtrl) * n_nets, H2D), "copy ctrl") &&
             hip_ok(hipMemcpy(h->d_step, &zero, sizeof(int64_t), H2D), "copy step") &&
             hip_ok(hipMemset(h->d_r_hist, 0, sizeof(float) * (size_t)lev::kRHistSteps * n_nets), "memset r") &&
             hip_ok(hipMemset(h->d_drive_u, 0, sizeof(float) * lev::kDriveSteps), "memset u") &&
-            hip_ok(hipMemset(h->d_partials, 0, sizeof(double2) * blocks.size()), "memset partials");
+            hip_ok(hipMemset(h->d_partials, 0, sizeof(double2) * std::max(h->n_partials, 1)), "memset partials");
   if (!ok) return fail();
+  for (int q = 0; q < 3; ++q) {
+    if (!h->plans[q].built) continue;
+    DevPlan& pl = h->plans[q];
+    if (!dalloc(h, &pl.d_items, hp[q].items.size()) || !dalloc(h, &pl.d_ranges, (size_t)n_nets)) return fail();
+    if (!hip_ok(hipMemcpy(pl.d_items, hp[q].items.data(), sizeof(int4) * hp[q].items.size(), H2D), "copy plan") ||
+        !hip_ok(hipMemcpy(pl.d_ranges, hp[q].ranges.data(), sizeof(int2) * n_nets, H2D), "copy ranges"))
+      return fail();
+  }
+  if (h->plans[kPlanBinned].built) {
+    DevPlan& pb = h->plans[kPlanBinned];
+    const size_t E = (size_t)h->E;
+    if (!dalloc(h, &pb.d_items, hb.buckets.items.size()) || !dalloc(h, &pb.d_ranges, (size_t)n_nets) ||
+        !dalloc(h, &pb.d_sources, hb.sources.size()) || !dalloc(h, &pb.d_sb_index, hb.sb_index.size()) ||
+        !dalloc(h, &pb.d_sb_bounds, hb.sb_bounds.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
+        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_perm, E) ||
+        !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes))
+      return fail();
+    ok = hip_ok(hipMemcpy(pb.d_items, hb.buckets.items.data(), sizeof(int4) * hb.buckets.items.size(), H2D),
+                "copy buckets") &&
+         hip_ok(hipMemcpy(pb.d_ranges, hb.buckets.ranges.data(), sizeof(int2) * n_nets, H2D), "copy ranges") &&
+         hip_ok(hipMemcpy(pb.d_sources, hb.sources.data(), sizeof(int4) * hb.sources.size(), H2D), "copy sources") &&
+         (hb.sb_index.empty() ||
+          hip_ok(hipMemcpy(pb.d_sb_index, hb.sb_index.data(), sizeof(int2) * hb.sb_index.size(), H2D), "copy sb")) &&
+         (hb.sb_bounds.empty() ||
+          hip_ok(hipMemcpy(pb.d_sb_bounds, hb.sb_bounds.data(), sizeof(int32_t) * hb.sb_bounds.size(), H2D),
+                 "copy bounds")) &&
+         hip_ok(hipMemcpy(h->d_csc_ptr, hb.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
+         (E == 0 || (hip_ok(hipMemcpy(h->d_delay_csc, hb.delay_csc.data(), E, H2D), "copy csc delays") &&
+                     hip_ok(hipMemcpy(h->d_pos, hb.pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
+                     hip_ok(hipMemcpy(h->d_perm, hb.perm.data(), sizeof(uint16_t) * E, H2D), "copy perm")));
+    if (!ok) return fail();
+  }
   // history pre-fill (init_buffer_kernel, leviathan_kernel.cu:371-372)
   if (!hip_ok(lev::launch_fill_ring(cfg.history, cfg.layout, h->d_ring, h->ring_pitch, h->n_slots, h->d_theta, N,
                                     h->stream),
@@ -540,6 +640,8 @@ void leviathan_config_default(LeviathanConfig* c) {
   c->layout = LEV_LAYOUT_NODE_MAJOR;
   c->edges_per_thread = 8;
   c->graph_chunk = 100;
+  c->schedule = LEV_SCHEDULE_DIRECT;
+  c->bucket_edges = 0;
 }
 
 LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr, const int* h_col_idx,
@@ -777,7 +879,7 @@ int leviathan_num_nodes(LeviathanHandle* h) { return h ? h->N : 0; }
 int64_t leviathan_num_edges(LeviathanHandle* h) { return h ? h->E : 0; }
 int64_t leviathan_step_count(LeviathanHandle* h) { return h ? h->step : 0; }
 int leviathan_num_nets(LeviathanHandle* h) { return h ? h->n_nets : 0; }
-int leviathan_num_blocks(LeviathanHandle* h) { return h ? h->num_blocks : 0; }
+int leviathan_num_blocks(LeviathanHandle* h) { return h ? h->plans[h->active].n_items : 0; }
 
 int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
   if (!check_handle(h)) return -1;
@@ -789,35 +891,41 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
     set_err("profile_steps runs undriven steps; switch the drive off first");
     return -1;
   }
-  std::vector<hipEvent_t> ev((size_t)2 * n + 1, nullptr);
+  const bool binned = h->plans[h->active].kind == LEV_SCHEDULE_BINNED;
+  // per step: [start, (mid: after scatter), after main, after finalize]
+  std::vector<hipEvent_t> ev((size_t)4 * n, nullptr);
   bool ok = true;
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) ok = false;
-  if (ok) {
-    const StepArgs a = step_args(h, dt);
-    const FinalizeArgs f = fin_args(h, dt);
-    ok = hipEventRecord(ev[0], h->stream) == hipSuccess;
-    for (int t = 0; t < n && ok; ++t) {
-      ok = lev::launch_step(h->cfg.history, h->cfg.layout, h->cfg.edges_per_thread, a, h->stream) == hipSuccess &&
-           hipEventRecord(ev[2 * t + 1], h->stream) == hipSuccess &&
-           lev::launch_finalize(f, h->stream) == hipSuccess && hipEventRecord(ev[2 * t + 2], h->stream) == hipSuccess;
-    }
-    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+  const FinalizeArgs f = fin_args(h, dt, h->plans[h->active]);
+  for (int t = 0; t < n && ok; ++t) {
+    ok = hipEventRecord(ev[4 * t], h->stream) == hipSuccess &&
+         enqueue_main(h, dt, binned ? ev[4 * t + 1] : nullptr) == hipSuccess &&
+         hipEventRecord(ev[4 * t + 2], h->stream) == hipSuccess &&
+         lev::launch_finalize(f, h->stream) == hipSuccess && hipEventRecord(ev[4 * t + 3], h->stream) == hipSuccess;
   }
+  ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
   if (ok) {
-    double ks = 0.0, kf = 0.0;
+    double kmain = 0.0, kfin = 0.0, kscat = 0.0;
     for (int t = 0; t < n; ++t) {
-      float a = 0.f, b = 0.f;
-      (void)hipEventElapsedTime(&a, ev[2 * t], ev[2 * t + 1]);
-      (void)hipEventElapsedTime(&b, ev[2 * t + 1], ev[2 * t + 2]);
-      ks += a;
-      kf += b;
+      float a = 0.f, b = 0.f, c = 0.f;
+      if (binned) {
+        (void)hipEventElapsedTime(&c, ev[4 * t], ev[4 * t + 1]);
+        (void)hipEventElapsedTime(&a, ev[4 * t + 1], ev[4 * t + 2]);
+      } else {
+        (void)hipEventElapsedTime(&a, ev[4 * t], ev[4 * t + 2]);
+      }
+      (void)hipEventElapsedTime(&b, ev[4 * t + 2], ev[4 * t + 3]);
+      kmain += a;
+      kfin += b;
+      kscat += c;
     }
     float wall = 0.f;
-    (void)hipEventElapsedTime(&wall, ev[0], ev[2 * n]);
-    out[0] = ks / n;
-    out[1] = kf / n;
+    (void)hipEventElapsedTime(&wall, ev[0], ev[4 * n - 1]);
+    out[0] = kmain / n;
+    out[1] = kfin / n;
     out[2] = wall;
+    out[3] = kscat / n;
     h->step += n;
   } else {
     set_err("profile_steps: a HIP call failed");
@@ -825,6 +933,112 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   return ok ? 0 : -1;
+}
+
+int leviathan_set_schedule(LeviathanHandle* h, int schedule, int ept) {
+  if (!check_handle(h)) return -1;
+  int idx;
+  if (schedule == LEV_SCHEDULE_BINNED) {
+    idx = kPlanBinned;
+  } else if (schedule == LEV_SCHEDULE_DIRECT && (ept == 4 || ept == 8 || ept == 16)) {
+    idx = plan_of_ept(ept);
+  } else {
+    set_err("schedule must be DIRECT (edges_per_thread 4, 8 or 16) or BINNED");
+    return -1;
+  }
+  if (!h->plans[idx].built) {
+    set_err("that schedule was not built for this handle (create it with LEV_SCHEDULE_AUTO)");
+    return -1;
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_err("hipStreamSynchronize failed");
+    return -1;
+  }
+  if (idx != h->active) drop_graphs(h);  // captured graphs hold the old plan's arguments
+  h->active = idx;
+  return 0;
+}
+
+int leviathan_get_schedule(LeviathanHandle* h, int* schedule, int* ept) {
+  if (!check_handle(h)) return -1;
+  if (schedule) *schedule = h->plans[h->active].kind;
+  if (ept) *ept = h->plans[h->active].kind == LEV_SCHEDULE_DIRECT ? h->plans[h->active].ept : 0;
+  return 0;
+}
+
+int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) {
+  if (!check_handle(h)) return -1;
+  if (steps <= 0) {
+    set_err("steps must be positive");
+    return -1;
+  }
+  if (h->drive_on) {
+    set_err("autotune runs undriven steps; switch the drive off first");
+    return -1;
+  }
+  // snapshot of everything a step changes
+  const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+  const size_t ring_bytes = (h->cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)h->N * h->ring_pitch
+                                                                      : (size_t)h->n_slots * h->ring_pitch) *
+                            hist_bytes;
+  struct Buf {
+    void* dev;
+    size_t bytes;
+    void* save;
+  } bufs[] = {{h->d_w, sizeof(float) * (size_t)h->E, nullptr},
+              {h->d_theta, sizeof(float) * (size_t)h->N, nullptr},
+              {h->d_theta_hat, sizeof(float) * (size_t)h->N, nullptr},
+              {h->d_omega_hat, sizeof(float) * (size_t)h->N, nullptr},
+              {h->d_ring, ring_bytes, nullptr},
+              {h->d_ctrl, sizeof(NetCtrl) * (size_t)h->n_nets, nullptr},
+              {h->d_step, sizeof(int64_t), nullptr}};
+  bool ok = hipStreamSynchronize(h->stream) == hipSuccess;
+  for (auto& b : bufs) {
+    if (!ok) break;
+    ok = hipMalloc(&b.save, std::max<size_t>(b.bytes, 1)) == hipSuccess &&
+         hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+  }
+  const int64_t step0 = h->step;
+  const int active0 = h->active;
+  double best = 1e300;
+  int best_idx = active0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (int q = 0; q < kNumPlans; ++q) {
+    if (ms_out) ms_out[q] = std::numeric_limits<double>::quiet_NaN();
+    if (!ok || !h->plans[q].built) continue;
+    h->active = q;
+    // restore the snapshot, warm up one step, time `steps` steps
+    for (auto& b : bufs)
+      ok = ok && hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+    ok = ok && enqueue_step(h, dt) && hipEventRecord(e0, h->stream) == hipSuccess;
+    for (int t = 0; t < steps && ok; ++t) ok = enqueue_step(h, dt);
+    ok = ok && hipEventRecord(e1, h->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0.f;
+    if (ok) (void)hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / steps;
+    if (ms_out) ms_out[q] = per;
+    if (ok && per < best) {
+      best = per;
+      best_idx = q;
+    }
+  }
+  // restore the state the call started from and keep the fastest plan
+  for (auto& b : bufs)
+    ok = ok && hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+  ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+  for (auto& b : bufs)
+    if (b.save) (void)hipFree(b.save);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  h->step = step0;
+  if (best_idx != active0) drop_graphs(h);
+  h->active = ok ? best_idx : active0;
+  if (!ok) {
+    set_err("autotune: a HIP call failed");
+    return -1;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -24,8 +24,6 @@ struct NetCtrl {
   double r0, beta, mu, g_rest, g_min, g_max;
   int32_t controller;
   int32_t n_nodes;
-  int32_t blk_begin;  // range of row blocks (and partials) of this network
-  int32_t blk_end;
 };
 
 template <class H>
@@ -62,9 +60,37 @@ struct FinalizeArgs {
   NetCtrl* ctrl;
   int64_t* step;
   float* r_hist;          // [kRHistSteps][n_nets]
+  const int2* ranges;     // per network: [begin, end) of its partials in the active plan
   int32_t n_nets;
-  int32_t num_blocks;
+  int32_t num_blocks;     // partials of the active plan
   float dt;
+};
+
+// Two-phase (binned) step: see leviathan_binned.hip.
+constexpr int kScatterEpt = 8;      // phase 1: out-edges per lane in a source block
+constexpr int kBucketSubEpt = 8;    // phase 2: edges per lane in a row sub-block
+constexpr int kMaxBucketLds = 48 * 1024;
+
+struct ScatterArgs {
+  const int32_t* csc_ptr;   // [N+1] out-edge offsets per source
+  const uint8_t* delay;     // [E] delays in CSC order
+  const uint32_t* pos;      // [E] position of each out-edge in the bucketed h array
+  const void* ring;
+  int64_t ring_pitch;
+  int32_t n_slots;
+  int32_t num_blocks;
+  int32_t xcd_remap;
+  const int4* blocks;       // {source_begin, source_end, 0, flags}
+  void* hbuf;               // [E] delayed phases, bucketed
+  const int64_t* step;
+};
+
+struct BucketArgs {
+  StepArgs s;               // s.blocks = buckets {row_begin, row_end, net, flags}
+  const void* hbuf;
+  const uint16_t* perm;     // [E] CSR edge -> position inside its bucket
+  const int2* sb_index;     // per bucket: {first bound, number of sub-blocks}
+  const int32_t* sb_bounds; // row boundaries of the sub-blocks
 };
 
 // Launchers (leviathan_kernels.hip).  Each returns the launch status.
@@ -74,5 +100,8 @@ hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, 
                             const float* theta, int32_t n, hipStream_t s);
 hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t pitch, int32_t slot,
                             float* out, int32_t n, hipStream_t s);
+// (leviathan_binned.hip)
+hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s);
+hipError_t launch_bucket(int history, int layout, const BucketArgs& a, size_t lds_bytes, hipStream_t s);
 
 }  // namespace lev

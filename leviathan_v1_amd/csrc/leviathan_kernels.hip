@@ -64,20 +64,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 
   if (!(blk.w & kBlockLongRow)) {
     // ---------------- stream mode ----------------
-    float th = 0.f, eps = 0.f, thh_new = 0.f, omh_new = 0.f;
+    RowState rs_;
     int32_t rs = 0, re = 0;
     if (tid < nrows) {
       const int32_t i = r0 + tid;
-      th = a.theta[i];
-      const float thh = a.theta_hat[i];
-      const float omh = a.omega_hat[i];
-      // prep (:101-109): eps from the wrapped prediction error, then the
-      // predictor advances with the old omega_hat.
-      eps = sinf(wrap_pm_pi_div(th - thh));
-      thh_new = thh + dt * omh;
-      omh_new = omh + a.kappa_w * dt * eps;
-      s_theta[tid] = th;
-      s_gate[tid] = 1.0f - a.gamma * fabsf(eps);
+      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
+      s_theta[tid] = rs_.th;
+      s_gate[tid] = rs_.gate;
       rs = a.row_ptr[i] - e0;
       re = a.row_ptr[i + 1] - e0;
       for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
@@ -111,16 +104,10 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) {
         const int lr = s_erow[le];
-        const float diff = wrap_pm_pi(hh[k] - s_theta[lr]);
-        float sn, cs;
-        sincosf(diff, &sn, &cs);
-        float wv = ww[k];
-        s_contrib[le] = wv * sn;  // coupling uses the pre-update weight (:154)
-        const float dw = dt * (a.eta * cs * s_gate[lr] - a.lam * wv);
-        wv += dw;
-        if (!__builtin_isfinite(wv)) wv = a.w_min;
-        wv = fmaxf(a.w_min, fminf(a.w_max, wv));
-        __builtin_nontemporal_store(wv, a.w + e0 + le);
+        float c;
+        const float wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
+        __builtin_nontemporal_store(wn, a.w + e0 + le);
       }
     }
     __syncthreads();
@@ -130,14 +117,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       CompSum acc;
       for (int32_t q = rs; q < re; ++q) acc.add(s_contrib[q]);
       float om = a.omega[i];
-      if (a.drive_b) om += a.drive_b[i] * u_t;
-      float x = th + dt * (om + g * acc.value() - a.alpha * eps);
-      x = wrap_0_2pi_div(x);
-      if (!__builtin_isfinite(x)) x = 0.0f;
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
+      const float x = row_integrate(rs_, om, g, acc.value(), dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
-      a.theta_hat[i] = thh_new;
-      a.omega_hat[i] = omh_new;
+      a.theta_hat[i] = rs_.thh_new;
+      a.omega_hat[i] = rs_.omh_new;
       float sx, cx;
       sincosf(x, &sx, &cx);
       my_c = (double)cx;
@@ -146,16 +131,11 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   } else {
     // ---------------- long-row mode: one row, whole workgroup ----------------
     const int32_t i = r0;
-    float th = 0.f, eps = 0.f, thh_new = 0.f, omh_new = 0.f;
+    RowState rs_;
     if (tid == 0) {
-      th = a.theta[i];
-      const float thh = a.theta_hat[i];
-      const float omh = a.omega_hat[i];
-      eps = sinf(wrap_pm_pi_div(th - thh));
-      thh_new = thh + dt * omh;
-      omh_new = omh + a.kappa_w * dt * eps;
-      s_theta[0] = th;
-      s_gate[0] = 1.0f - a.gamma * fabsf(eps);
+      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
+      s_theta[0] = rs_.th;
+      s_gate[0] = rs_.gate;
     }
     __syncthreads();
     const float th_i = s_theta[0];
@@ -186,16 +166,10 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int k = 0; k < 4; ++k) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
-          const float diff = wrap_pm_pi(hh[k] - th_i);
-          float sn, cs;
-          sincosf(diff, &sn, &cs);
-          float wv = ww[k];
-          acc.add(wv * sn);
-          const float dw = dt * (a.eta * cs * gate - a.lam * wv);
-          wv += dw;
-          if (!__builtin_isfinite(wv)) wv = a.w_min;
-          wv = fmaxf(a.w_min, fminf(a.w_max, wv));
-          __builtin_nontemporal_store(wv, a.w + e0 + le);
+          float c;
+          const float wn = edge_update(hh[k], th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+          acc.add(c);
+          __builtin_nontemporal_store(wn, a.w + e0 + le);
         }
       }
     }
@@ -208,14 +182,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 #pragma unroll
       for (int q = 0; q < kBlock / 64; ++q) tot += s_red[0][q];
       float om = a.omega[i];
-      if (a.drive_b) om += a.drive_b[i] * u_t;
-      float x = th + dt * (om + g * (float)tot - a.alpha * eps);
-      x = wrap_0_2pi_div(x);
-      if (!__builtin_isfinite(x)) x = 0.0f;
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
+      const float x = row_integrate(rs_, om, g, (float)tot, dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
-      a.theta_hat[i] = thh_new;
-      a.omega_hat[i] = omh_new;
+      a.theta_hat[i] = rs_.thh_new;
+      a.omega_hat[i] = rs_.omh_new;
       float sx, cx;
       sincosf(x, &sx, &cx);
       my_c = (double)cx;
@@ -290,7 +262,8 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
     }
   } else {
     for (int net = wave; net < a.n_nets; net += kFinalizeThreads / 64) {
-      const int32_t lo = a.ctrl[net].blk_begin, hi = a.ctrl[net].blk_end;
+      const int2 rg = a.ranges[net];
+      const int32_t lo = rg.x, hi = rg.y;
       double c = 0.0, s = 0.0;
       for (int32_t q = lo + lane; q < hi; q += 64) {
         const double2 p = a.partials[q];

@@ -1,0 +1,207 @@
+// plan.cpp -- host-side construction of the step's work decomposition.
+#include "plan.h"
+
+#include <algorithm>
+#include <thread>
+
+#include "leviathan_internal.h"
+
+namespace lev {
+
+namespace {
+
+inline int deg_of(const int* row_ptr, int r) { return row_ptr[r + 1] - row_ptr[r]; }
+
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([&f, lo, hi]() {
+      for (int64_t i = lo; i < hi; ++i) f(i);
+    });
+  }
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap, HostPlan& out, std::string& err) {
+  out.items.clear();
+  out.ranges.assign(n_nets, make_int2(0, 0));
+  for (int net = 0; net < n_nets; ++net) {
+    const int lo = offs[net], hi = offs[net + 1];
+    std::vector<int4> longs, shorts;
+    int r = lo;
+    while (r < hi) {
+      if (deg_of(row_ptr, r) > cap) {
+        longs.push_back(make_int4(r, r + 1, net, kBlockLongRow));
+        ++r;
+        continue;
+      }
+      const int start = r;
+      int nnz = 0;
+      while (r < hi && r - start < kBlock) {
+        const int d = deg_of(row_ptr, r);
+        if (d > cap || nnz + d > cap) break;
+        nnz += d;
+        ++r;
+      }
+      shorts.push_back(make_int4(start, r, net, 0));
+    }
+    std::stable_sort(longs.begin(), longs.end(),
+                     [&](const int4& a, const int4& b) { return deg_of(row_ptr, a.x) > deg_of(row_ptr, b.x); });
+    out.ranges[net].x = (int)out.items.size();
+    out.items.insert(out.items.end(), longs.begin(), longs.end());
+    out.items.insert(out.items.end(), shorts.begin(), shorts.end());
+    out.ranges[net].y = (int)out.items.size();
+  }
+  if (out.items.size() > (size_t)INT32_MAX) {
+    err = "too many row blocks";
+    return false;
+  }
+  return true;
+}
+
+bool build_binned_plan(int N, const int* row_ptr, const int* col, const uint8_t* delays, int n_nets,
+                       const int* offs, int bucket_edges, HostBinned& out, std::string& err) {
+  const int cap_sub = kBlock * kBucketSubEpt;
+  const int64_t E = row_ptr[N];
+  if (bucket_edges < cap_sub || bucket_edges > 65536) {
+    err = "bucket_edges must be in [" + std::to_string(cap_sub) + ", 65536]";
+    return false;
+  }
+  // 1. buckets and their row sub-blocks
+  HostPlan& bk = out.buckets;
+  bk.items.clear();
+  bk.ranges.assign(n_nets, make_int2(0, 0));
+  out.sb_index.clear();
+  out.sb_bounds.clear();
+  out.max_bucket_edges = 0;
+  for (int net = 0; net < n_nets; ++net) {
+    const int lo = offs[net], hi = offs[net + 1];
+    std::vector<int4> longs, shorts;
+    std::vector<std::vector<int32_t>> shorts_b;
+    int r = lo;
+    while (r < hi) {
+      if (deg_of(row_ptr, r) > cap_sub) {
+        longs.push_back(make_int4(r, r + 1, net, kBlockLongRow));
+        ++r;
+        continue;
+      }
+      const int start = r;
+      int nnz = 0;
+      while (r < hi) {
+        const int d = deg_of(row_ptr, r);
+        if (d > cap_sub || nnz + d > bucket_edges) break;
+        nnz += d;
+        ++r;
+      }
+      out.max_bucket_edges = std::max(out.max_bucket_edges, nnz);
+      std::vector<int32_t> b{start};
+      for (int s = start; s < r;) {
+        int t = s, sn = 0;
+        while (t < r && t - s < kBlock && sn + deg_of(row_ptr, t) <= cap_sub) {
+          sn += deg_of(row_ptr, t);
+          ++t;
+        }
+        b.push_back(t);
+        s = t;
+      }
+      shorts.push_back(make_int4(start, r, net, 0));
+      shorts_b.push_back(std::move(b));
+    }
+    std::stable_sort(longs.begin(), longs.end(),
+                     [&](const int4& a, const int4& b) { return deg_of(row_ptr, a.x) > deg_of(row_ptr, b.x); });
+    bk.ranges[net].x = (int)bk.items.size();
+    for (const int4& it : longs) {
+      bk.items.push_back(it);
+      out.sb_index.push_back(make_int2((int)out.sb_bounds.size(), 0));
+    }
+    for (size_t q = 0; q < shorts.size(); ++q) {
+      bk.items.push_back(shorts[q]);
+      out.sb_index.push_back(make_int2((int)out.sb_bounds.size(), (int)shorts_b[q].size() - 1));
+      out.sb_bounds.insert(out.sb_bounds.end(), shorts_b[q].begin(), shorts_b[q].end());
+    }
+    bk.ranges[net].y = (int)bk.items.size();
+  }
+
+  // 2. position of every CSR edge inside its bucket: buckets are ordered by
+  //    (source, CSR index); single-row buckets keep CSR order (= source order).
+  std::vector<uint32_t> pos_of_e((size_t)E);
+  out.perm.assign((size_t)E, 0);
+  const int64_t nb = (int64_t)bk.items.size();
+  parallel_for(nb, [&](int64_t q) {
+    const int4 it = bk.items[q];
+    const int32_t e0 = row_ptr[it.x], e1 = row_ptr[it.y];
+    const int32_t n = e1 - e0;
+    if (it.w & kBlockLongRow) {
+      for (int32_t l = 0; l < n; ++l) pos_of_e[e0 + l] = (uint32_t)(e0 + l);
+      return;
+    }
+    std::vector<uint64_t> key((size_t)n);
+    for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)(uint32_t)col[e0 + l] << 32) | (uint32_t)l;
+    std::sort(key.begin(), key.end());
+    for (int32_t k = 0; k < n; ++k) {
+      const uint32_t l = (uint32_t)(key[k] & 0xffffffffu);
+      out.perm[e0 + l] = (uint16_t)k;
+      pos_of_e[e0 + l] = (uint32_t)(e0 + k);
+    }
+  });
+
+  // 3. CSC (by source), destinations ascending within a source
+  out.csc_ptr.assign((size_t)N + 1, 0);
+  for (int64_t e = 0; e < E; ++e) out.csc_ptr[col[e] + 1]++;
+  for (int j = 0; j < N; ++j) out.csc_ptr[j + 1] += out.csc_ptr[j];
+  std::vector<int32_t> cur(out.csc_ptr.begin(), out.csc_ptr.end() - 1);
+  out.pos.assign((size_t)E, 0);
+  out.delay_csc.assign((size_t)E, 0);
+  for (int64_t e = 0; e < E; ++e) {
+    const int32_t c = cur[col[e]]++;
+    out.pos[c] = pos_of_e[e];
+    out.delay_csc[c] = delays[e];
+  }
+  // every h slot is written exactly once per step: pos is a permutation of [0, E)
+  {
+    std::vector<uint8_t> seen((size_t)E, 0);
+    for (int64_t c = 0; c < E; ++c) {
+      if (out.pos[c] >= (uint64_t)E || seen[out.pos[c]]) {
+        err = "internal error: bucket positions are not a permutation";
+        return false;
+      }
+      seen[out.pos[c]] = 1;
+    }
+  }
+
+  // 4. phase-1 work items over sources
+  const int cap_src = kBlock * kScatterEpt;
+  out.sources.clear();
+  std::vector<int4> longs, shorts;
+  int s = 0;
+  while (s < N) {
+    const int d = out.csc_ptr[s + 1] - out.csc_ptr[s];
+    if (d > cap_src) {
+      longs.push_back(make_int4(s, s + 1, 0, kBlockLongRow));
+      ++s;
+      continue;
+    }
+    const int start = s;
+    int nnz = 0;
+    while (s < N && s - start < kBlock) {
+      const int dd = out.csc_ptr[s + 1] - out.csc_ptr[s];
+      if (dd > cap_src || nnz + dd > cap_src) break;
+      nnz += dd;
+      ++s;
+    }
+    shorts.push_back(make_int4(start, s, 0, 0));
+  }
+  out.sources.insert(out.sources.end(), longs.begin(), longs.end());
+  out.sources.insert(out.sources.end(), shorts.begin(), shorts.end());
+  return true;
+}
+
+}  // namespace lev

@@ -19,16 +19,16 @@ constexpr float kRadToU16 = 10430.3784f;      // 65536 / (2*pi)
 // (:146); a multiply by the rounded reciprocal changes only which side of the
 // +-pi seam a value lands on, which sin/cos do not see.
 __device__ __forceinline__ float wrap_pm_pi(float x) {
-  return x - floorf(x * kInvTwoPi + 0.5f) * kTwoPi;
+  return fmaf(-floorf(fmaf(x, kInvTwoPi, 0.5f)), kTwoPi, x);
 }
 
 // Exactly the reference's per-node wraps (true division): prep's [-pi, pi)
 // (:105) and integrate's [0, 2*pi) (:188).
 __device__ __forceinline__ float wrap_pm_pi_div(float x) {
-  return x - floorf(x / kTwoPi + 0.5f) * kTwoPi;
+  return fmaf(-floorf(x / kTwoPi + 0.5f), kTwoPi, x);
 }
 __device__ __forceinline__ float wrap_0_2pi_div(float x) {
-  return x - floorf(x / kTwoPi) * kTwoPi;
+  return fmaf(-floorf(x / kTwoPi), kTwoPi, x);
 }
 
 template <class H>
@@ -84,6 +84,47 @@ struct CompSum {
   }
   __device__ __forceinline__ float value() const { return s + c; }
 };
+
+// Per-row part of the step, shared by every kernel variant so the arithmetic
+// is written once.  The library is compiled with -ffp-contract=off and every
+// fused multiply-add is explicit, so all kernel variants round identically.
+// prep (leviathan_kernel.cu:101-109): eps from the wrapped prediction error;
+// the predictor advances with the OLD omega_hat.
+struct RowState {
+  float th, eps, thh_new, omh_new, gate;
+};
+
+__device__ __forceinline__ RowState row_prep(float th, float thh, float omh, float dt, float kappa_w, float gamma) {
+  RowState s;
+  s.th = th;
+  s.eps = sinf(wrap_pm_pi_div(th - thh));
+  s.thh_new = fmaf(dt, omh, thh);
+  s.omh_new = fmaf(kappa_w * dt, s.eps, omh);
+  s.gate = fmaf(-gamma, fabsf(s.eps), 1.0f);
+  return s;
+}
+
+// integrate (leviathan_kernel.cu:183-190): wrap to [0, 2*pi), NaN -> 0.
+__device__ __forceinline__ float row_integrate(const RowState& s, float om, float g, float coupling, float dt,
+                                               float alpha) {
+  float x = fmaf(dt, fmaf(-alpha, s.eps, fmaf(g, coupling, om)), s.th);
+  x = wrap_0_2pi_div(x);
+  if (!__builtin_isfinite(x)) x = 0.0f;
+  return x;
+}
+
+// Hebbian update of one edge (leviathan_kernel.cu:157-163); returns the new
+// weight and writes the coupling contribution w_old * sin(delta).
+__device__ __forceinline__ float edge_update(float h, float th_i, float gate, float w, float dt, float eta, float lam,
+                                             float w_min, float w_max, float* contrib) {
+  const float diff = wrap_pm_pi(h - th_i);
+  float sn, cs;
+  sincosf(diff, &sn, &cs);
+  *contrib = w * sn;
+  float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
+  if (!__builtin_isfinite(wv)) wv = w_min;
+  return fmaxf(w_min, fminf(w_max, wv));
+}
 
 // Logical row-block index for a workgroup.  Workgroups are dealt to the 8
 // XCDs round-robin; this mapping gives each XCD one contiguous run of row

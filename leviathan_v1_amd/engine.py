@@ -48,6 +48,10 @@ class LeviathanEngine:
       history: "f32" (default) or "u16" (exact-wrap uint16 angles)
       layout:  "node" (ring[node][slot], default) or "slot" (ring[slot][node])
       edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
+      schedule: "direct" (one fused kernel, per-edge gather), "binned" (source-
+               ordered scatter into destination buckets, then the fused row
+               update) or "auto" (both built; autotune() picks)
+      bucket_edges: binned schedule bucket capacity (0: automatic)
       graph_chunk: steps per captured HIP graph in step_n (0 disables graphs)
       params: dict of physics parameters (names of LeviathanConfig)
       n_nets, net_offsets, g0, controller: ensemble of independent networks
@@ -55,6 +59,7 @@ class LeviathanEngine:
 
     def __init__(self, N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega, *,
                  history: str = "f32", layout: str = "node", edges_per_thread: int = 8, graph_chunk: int = 100,
+                 schedule: str = "direct", bucket_edges: int = 0,
                  params: Optional[dict] = None, n_nets: int = 1, net_offsets: Optional[Sequence[int]] = None,
                  g0: Optional[Sequence[float]] = None, controller: Optional[Sequence[bool]] = None):
         self._h = None
@@ -73,6 +78,7 @@ class LeviathanEngine:
         self.N, self.E, self.max_delay = N, E, int(max_delay)
         self.n_nets = int(n_nets)
         extended = (history != "f32" or layout != "node" or edges_per_thread != 8 or graph_chunk != 100
+                    or schedule != "direct" or bucket_edges
                     or params or n_nets != 1 or net_offsets is not None or g0 is not None
                     or controller is not None)
         if not extended:
@@ -88,6 +94,10 @@ class LeviathanEngine:
             cfg.layout = _lib.LAYOUT[layout]
             cfg.edges_per_thread = int(edges_per_thread)
             cfg.graph_chunk = int(graph_chunk)
+            if schedule not in _lib.SCHEDULE:
+                raise ValueError(f"schedule must be one of {list(_lib.SCHEDULE)}")
+            cfg.schedule = _lib.SCHEDULE[schedule]
+            cfg.bucket_edges = int(bucket_edges)
             for k, v in (params or {}).items():
                 if k == "gain_controller":
                     cfg.gain_controller = int(bool(v))
@@ -216,11 +226,32 @@ class LeviathanEngine:
         return int(_lib.lib().leviathan_num_blocks(self._h))
 
     def profile_steps(self, dt: float, n: int) -> dict:
-        """Times n eager steps with HIP events on the engine's stream."""
-        out = np.zeros(3, dtype=np.float64)
+        """Times n eager steps with HIP events on the engine's stream (ms per launch)."""
+        out = np.zeros(4, dtype=np.float64)
         if _lib.lib().leviathan_profile_steps(self._h, float(dt), int(n), _p(out)) != 0:
             _lib.check("profile_steps")
-        return {"step_kernel_ms": float(out[0]), "finalize_kernel_ms": float(out[1]), "wall_ms": float(out[2])}
+        return {"step_kernel_ms": float(out[0]), "finalize_kernel_ms": float(out[1]), "wall_ms": float(out[2]),
+                "scatter_kernel_ms": float(out[3])}
+
+    def set_schedule(self, schedule: str, edges_per_thread: int = 8) -> None:
+        if _lib.lib().leviathan_set_schedule(self._h, _lib.SCHEDULE[schedule], int(edges_per_thread)) != 0:
+            _lib.check("set_schedule")
+
+    def get_schedule(self):
+        s = ctypes.c_int(0)
+        e = ctypes.c_int(0)
+        if _lib.lib().leviathan_get_schedule(self._h, ctypes.byref(s), ctypes.byref(e)) != 0:
+            _lib.check("get_schedule")
+        return ({v: k for k, v in _lib.SCHEDULE.items()}[s.value], e.value)
+
+    def autotune(self, dt: float, steps: int = 10) -> dict:
+        """Times every built schedule from the current state (which is restored) and keeps the fastest.
+        Returns ms per step of each candidate."""
+        out = np.zeros(4, dtype=np.float64)
+        if _lib.lib().leviathan_autotune(self._h, float(dt), int(steps), _p(out)) != 0:
+            _lib.check("autotune")
+        names = ["direct/4", "direct/8", "direct/16", "binned"]
+        return {n: float(v) for n, v in zip(names, out) if v == v}
 
     def close(self) -> None:
         if self._h is not None:

@@ -325,3 +325,104 @@ def test_observatory_surface():
     obs.set_phase_snapshot(snap.astype(np.float64))
     assert "TOTAL (Measured)" in obs.get_vram_report()
     assert obs.get_vram_usage() >= obs.nnz * 9 + 3000 * (52 * 4 + 16)
+
+
+# ---------------------------------------------------------------------------
+# binned (two-phase) schedule
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def ba200k():
+    from leviathan_v1_amd import workloads
+
+    wl = workloads.barabasi_albert_workload(200_000, 10, 50, seed=4)
+    assert wl.graph.degrees().max() > 2048  # exercises single-row buckets and long sources
+    return wl
+
+
+@pytest.mark.parametrize("history", ["f32", "u16"])
+@pytest.mark.parametrize("layout", ["node", "slot"])
+def test_binned_equals_direct(ba200k, history, layout):
+    """The binned schedule shares the direct step's per-row arithmetic and summation
+    order: phases and weights agree to FP32 rounding after 40 steps."""
+    say("binned vs direct", history, layout)
+    d = make(ba200k, history=history, layout=layout)
+    b = make(ba200k, history=history, layout=layout, schedule="binned")
+    assert b.get_schedule() == ("binned", 0)
+    rd = d.step_n(DT, 40)
+    rb = b.step_n(DT, 40)
+    assert np.abs(rd - rb).max() < 1e-6
+    assert theta_rel_err(b.get_theta(), d.get_theta()) < 1e-6
+    assert np.abs(b.get_weights() - d.get_weights()).max() < 1e-6
+
+
+@pytest.mark.parametrize("bucket_edges", [0, 2048])
+def test_binned_one_step_consistency(ws1k, bucket_edges):
+    """Binned schedule (default buckets and minimum-size buckets) against the FP64 oracle."""
+    say("binned one-step", bucket_edges)
+    eng = make(ws1k, schedule="binned", bucket_edges=bucket_edges, history="u16", layout="slot")
+    eng.step_n(DT, 33)
+    o = oracle_from_engine(eng, ws1k)
+    o.step(DT)
+    eng.step(DT)
+    assert theta_rel_err(eng.get_theta(), o.theta) < TOL
+    assert w_rel_err(eng.get_weights(), o.w) < TOL
+
+
+def test_binned_trajectory_vs_golden(ws1k, golden1k):
+    say("binned trajectory")
+    eng = make(ws1k, schedule="binned")
+    eng.step_n(DT, 10)
+    assert theta_rel_err(eng.get_theta(), golden1k["theta_10"]) < TOL
+    assert w_rel_err(eng.get_weights(), golden1k["w_10"]) < TOL
+
+
+def test_autotune_restores_state_and_switches(ba200k):
+    """autotune() times every schedule from the current state, restores it, keeps the fastest."""
+    say("autotune")
+    a = make(ba200k, schedule="auto")
+    ref = make(ba200k)
+    a.step_n(DT, 5)
+    ref.step_n(DT, 5)
+    times = a.autotune(DT, steps=3)
+    say("autotune ms/step", times)
+    assert set(times) == {"direct/4", "direct/8", "direct/16", "binned"}
+    assert a.step_count == 5
+    assert np.array_equal(a.get_theta(), ref.get_theta())
+    assert np.array_equal(a.get_weights(), ref.get_weights())
+    best = min(times, key=times.get)
+    sched, ept = a.get_schedule()
+    assert (sched == "binned") if best == "binned" else (sched, ept) == ("direct", int(best.split("/")[1]))
+    ra = a.step_n(DT, 20)
+    rr = ref.step_n(DT, 20)
+    assert np.abs(ra - rr).max() < 1e-6
+    assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
+    for s, e in [("direct", 4), ("binned", 0), ("direct", 16)]:
+        a.set_schedule(s, e or 8)
+        a.step_n(DT, 3)
+        ref.step_n(DT, 3)
+    assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
+    with pytest.raises(RuntimeError):
+        ref.set_schedule("binned")
+
+
+def test_binned_ensemble_equals_separate():
+    """Ensemble in the binned schedule == separate direct engines."""
+    from leviathan_v1_amd import LeviathanEngine, graph, workloads
+
+    say("binned ensemble")
+    sizes = [900, 2500, 1300]
+    wls = [workloads.watts_strogatz_workload(n, 20, 0.1, 50, seed=30 + i) for i, n in enumerate(sizes)]
+    g, offs = graph.disjoint_union([w.graph for w in wls])
+    cat = lambda f: np.concatenate([getattr(w, f) for w in wls])
+    g0 = [0.3, 2.0, 3.5]
+    ens = LeviathanEngine(g.num_nodes, 50, g.row_ptr, g.col_idx, cat("delays"), cat("weights"), cat("theta"),
+                          cat("theta_hat"), cat("omega"), n_nets=3, net_offsets=offs, g0=g0,
+                          controller=[False] * 3, schedule="binned", bucket_edges=4096)
+    r_ens = ens.step_n(DT, 60)
+    th = ens.get_theta()
+    for k, w in enumerate(wls):
+        e = LeviathanEngine(*w.engine_args(), params=dict(g0=g0[k], gain_controller=False))
+        r = e.step_n(DT, 60)
+        assert np.abs(r - r_ens[:, k]).max() < 1e-6, k
+        assert theta_rel_err(e.get_theta(), th[offs[k]:offs[k + 1]]) < 1e-6, k
