@@ -60,12 +60,12 @@ def make_workload(name, n_override=None):
     raise ValueError(name)
 
 
-def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", profile_steps=20):
+def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", bucket_edges=0, profile_steps=20):
     from leviathan_v1_amd import LeviathanEngine
 
     t0 = time.perf_counter()
     eng = LeviathanEngine(*wl.engine_args(), history=history, layout=layout, edges_per_thread=ept,
-                          schedule=schedule)
+                          schedule=schedule, bucket_edges=bucket_edges)
     log(f"{history}: engine ready in {time.perf_counter() - t0:.1f} s")
     try:
         tuned = eng.autotune(wl.dt, steps=5) if schedule == "auto" else None
@@ -154,6 +154,7 @@ def main():
     ap.add_argument("--layout", default="node", choices=["node", "slot"])
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned"])
+    ap.add_argument("--bucket-edges", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="f32,u16")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -176,7 +177,7 @@ def main():
     log(f"workload {wl.name}: N={wl.N} E={wl.E} built in {time.perf_counter() - t0:.1f} s")
     forms = {}
     for h in args.forms.split(","):
-        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept, args.schedule)
+        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept, args.schedule, args.bucket_edges)
         log(h, json.dumps(forms[h]))
     head = forms.get("u16") or forms["f32"]
     hist = head["history"]

@@ -165,14 +165,15 @@ int leviathan_num_blocks(LeviathanHandle* handle);
 int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* out);
 
 /* Schedule control (handles created with LEV_SCHEDULE_BINNED or _AUTO hold
- * both plans).  leviathan_set_schedule switches between them;
- * leviathan_autotune times `steps` steps of every candidate (direct with 4, 8
- * and 16 edges per lane, binned) from the same state, restores the state,
- * keeps the fastest and writes the mean milliseconds per step of each
- * candidate to ms_out[4] (NaN for a candidate not available).  The
- * schedules give the same phases and weights (the per-row arithmetic and
- * summation order are shared); r differs only in the grouping of its FP64
- * partial sums.  Return 0 or -1. */
+ * every plan: direct and binned, each with 4, 8 and 16 edges per lane in the
+ * row pass).  leviathan_set_schedule switches between them;
+ * leviathan_autotune times `steps` steps of every built plan from the same
+ * state, restores the state, keeps the fastest and writes the mean
+ * milliseconds per step to ms_out[6] (direct/4, direct/8, direct/16,
+ * binned/4, binned/8, binned/16; NaN for a plan not built).  All plans give
+ * the same phases and weights (the per-row arithmetic and summation order
+ * are shared); r differs only in the grouping of its FP64 partial sums.
+ * Return 0 or -1. */
 int leviathan_set_schedule(LeviathanHandle* handle, int schedule, int edges_per_thread);
 int leviathan_get_schedule(LeviathanHandle* handle, int* schedule, int* edges_per_thread);
 int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_out);

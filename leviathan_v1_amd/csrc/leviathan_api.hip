@@ -51,7 +51,8 @@ struct GraphEntry {
 };
 
 // One schedule's work decomposition on the device.
-constexpr int kPlanDirect4 = 0, kPlanDirect8 = 1, kPlanDirect16 = 2, kPlanBinned = 3, kNumPlans = 4;
+// Plans 0-2: direct with 4, 8, 16 edges per lane; 3-5: binned, same row blocks.
+constexpr int kPlanDirect4 = 0, kPlanDirect8 = 1, kPlanDirect16 = 2, kPlanBinned = 3, kNumPlans = 6;
 
 struct DevPlan {
   bool built = false;
@@ -60,15 +61,11 @@ struct DevPlan {
   int4* d_items = nullptr;
   int32_t n_items = 0;
   int2* d_ranges = nullptr;
-  // binned only
-  int4* d_sources = nullptr;
-  int32_t n_sources = 0;
-  int2* d_sb_index = nullptr;
-  int32_t* d_sb_bounds = nullptr;
-  size_t lds = 0;
+  bool owns = true;  // binned plans share the row blocks of the direct plan with the same ept
 };
 
 int plan_of_ept(int ept) { return ept == 4 ? kPlanDirect4 : ept == 16 ? kPlanDirect16 : kPlanDirect8; }
+const char* const kPlanNames[kNumPlans] = {"direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"};
 
 }  // namespace
 
@@ -102,7 +99,10 @@ struct LeviathanHandle {
   uint8_t* d_delay_csc = nullptr;
   uint32_t* d_pos = nullptr;
   void* d_hbuf = nullptr;
-  uint16_t* d_perm = nullptr;
+  uint32_t* d_hpos = nullptr;
+  int4* d_sources = nullptr;  // phase-1 work items
+  int32_t n_sources = 0;
+  int32_t n_buckets = 0;
   NetCtrl* d_ctrl = nullptr;
   int64_t* d_step = nullptr;
   float* d_r_hist = nullptr;
@@ -145,11 +145,12 @@ void destroy(LeviathanHandle* h) {
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
-                  h->d_pos,     h->d_hbuf,      h->d_perm};
+                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_sources};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
-    void* pp[] = {pl.d_items, pl.d_ranges, pl.d_sources, pl.d_sb_index, pl.d_sb_bounds};
+    if (!pl.owns) continue;
+    void* pp[] = {pl.d_items, pl.d_ranges};
     for (void* p : pp)
       if (p) (void)hipFree(p);
   }
@@ -192,6 +193,8 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.step = h->d_step;
   a.drive_b = h->drive_on ? h->d_drive_b : nullptr;
   a.drive_u = h->d_drive_u;
+  a.hbuf = h->d_hbuf;
+  a.hpos = h->d_hpos;
   a.dt = dt;
   a.eta = h->cfg.eta;
   a.lam = h->cfg.lam;
@@ -224,31 +227,23 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   sa.ring = h->d_ring;
   sa.ring_pitch = h->ring_pitch;
   sa.n_slots = h->n_slots;
-  sa.num_blocks = pl.n_sources;
+  (void)pl;
+  sa.num_blocks = h->n_sources;
   sa.xcd_remap = 1;
-  sa.blocks = pl.d_sources;
+  sa.blocks = h->d_sources;
   sa.hbuf = h->d_hbuf;
   sa.step = h->d_step;
   return sa;
 }
 
-lev::BucketArgs bucket_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
-  lev::BucketArgs ba{};
-  ba.s = step_args(h, dt, pl);
-  ba.hbuf = h->d_hbuf;
-  ba.perm = h->d_perm;
-  ba.sb_index = pl.d_sb_index;
-  ba.sb_bounds = pl.d_sb_bounds;
-  return ba;
-}
-
 // Enqueue the step kernel(s) of the active plan, without the finalize.
 hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr) {
   const DevPlan& pl = h->plans[h->active];
-  if (pl.kind == LEV_SCHEDULE_DIRECT) return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, step_args(h, dt, pl), h->stream);
+  if (pl.kind == LEV_SCHEDULE_DIRECT)
+    return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
   hipError_t e = lev::launch_scatter(h->cfg.history, h->cfg.layout, scatter_args(h, pl), h->stream);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
-  if (e == hipSuccess) e = lev::launch_bucket(h->cfg.history, h->cfg.layout, bucket_args(h, dt, pl), pl.lds, h->stream);
+  if (e == hipSuccess) e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 1, step_args(h, dt, pl), h->stream);
   return e;
 }
 
@@ -413,12 +408,10 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     return nullptr;
   }
   const size_t hist_bytes = cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  if (cfg.bucket_edges == 0) cfg.bucket_edges = (int)(32768 / hist_bytes);
-  if (cfg.schedule != LEV_SCHEDULE_DIRECT &&
-      (cfg.bucket_edges < lev::kBlock * lev::kBucketSubEpt || cfg.bucket_edges > 65536 ||
-       (size_t)cfg.bucket_edges * hist_bytes > (size_t)lev::kMaxBucketLds)) {
-    set_err("bucket_edges must be in [%d, %d] for this history format", lev::kBlock * lev::kBucketSubEpt,
-            (int)std::min<size_t>(65536, lev::kMaxBucketLds / hist_bytes));
+  // Default bucket: 1 MB of delayed phases, a quarter of one XCD's L2.
+  if (cfg.bucket_edges == 0) cfg.bucket_edges = (int)((1 << 20) / hist_bytes);
+  if (cfg.schedule != LEV_SCHEDULE_DIRECT && cfg.bucket_edges < 1024) {
+    set_err("bucket_edges must be at least 1024");
     return nullptr;
   }
   if (!validate_inputs(N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega)) return nullptr;
@@ -510,14 +503,18 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       set_err("%s", perr.c_str());
       return fail();
     }
-    DevPlan& pb = h->plans[kPlanBinned];
-    pb.built = true;
-    pb.kind = LEV_SCHEDULE_BINNED;
-    pb.n_items = (int32_t)hb.buckets.items.size();
-    pb.n_sources = (int32_t)hb.sources.size();
-    pb.lds = (size_t)std::max(hb.max_bucket_edges, 1) * hist_bytes;
+    for (int q = 0; q < 3; ++q) {
+      DevPlan& pb = h->plans[kPlanBinned + q];
+      pb.built = true;
+      pb.kind = LEV_SCHEDULE_BINNED;
+      pb.ept = epts[q];
+      pb.n_items = h->plans[q].n_items;
+      pb.owns = false;
+    }
+    h->n_sources = (int32_t)hb.sources.size();
+    h->n_buckets = hb.n_buckets;
   }
-  h->active = cfg.schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : plan_of_ept(cfg.edges_per_thread);
+  h->active = (cfg.schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : 0) + plan_of_ept(cfg.edges_per_thread);
   for (const auto& pl : h->plans)
     if (pl.built) h->n_partials = std::max(h->n_partials, pl.n_items);
   for (int k = 0; k < n_nets; ++k) {
@@ -574,27 +571,20 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       return fail();
   }
   if (h->plans[kPlanBinned].built) {
-    DevPlan& pb = h->plans[kPlanBinned];
+    for (int q = 0; q < 3; ++q) {
+      h->plans[kPlanBinned + q].d_items = h->plans[q].d_items;
+      h->plans[kPlanBinned + q].d_ranges = h->plans[q].d_ranges;
+    }
     const size_t E = (size_t)h->E;
-    if (!dalloc(h, &pb.d_items, hb.buckets.items.size()) || !dalloc(h, &pb.d_ranges, (size_t)n_nets) ||
-        !dalloc(h, &pb.d_sources, hb.sources.size()) || !dalloc(h, &pb.d_sb_index, hb.sb_index.size()) ||
-        !dalloc(h, &pb.d_sb_bounds, hb.sb_bounds.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
-        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_perm, E) ||
+    if (!dalloc(h, &h->d_sources, hb.sources.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
+        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) ||
         !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes))
       return fail();
-    ok = hip_ok(hipMemcpy(pb.d_items, hb.buckets.items.data(), sizeof(int4) * hb.buckets.items.size(), H2D),
-                "copy buckets") &&
-         hip_ok(hipMemcpy(pb.d_ranges, hb.buckets.ranges.data(), sizeof(int2) * n_nets, H2D), "copy ranges") &&
-         hip_ok(hipMemcpy(pb.d_sources, hb.sources.data(), sizeof(int4) * hb.sources.size(), H2D), "copy sources") &&
-         (hb.sb_index.empty() ||
-          hip_ok(hipMemcpy(pb.d_sb_index, hb.sb_index.data(), sizeof(int2) * hb.sb_index.size(), H2D), "copy sb")) &&
-         (hb.sb_bounds.empty() ||
-          hip_ok(hipMemcpy(pb.d_sb_bounds, hb.sb_bounds.data(), sizeof(int32_t) * hb.sb_bounds.size(), H2D),
-                 "copy bounds")) &&
+    ok = hip_ok(hipMemcpy(h->d_sources, hb.sources.data(), sizeof(int4) * hb.sources.size(), H2D), "copy sources") &&
          hip_ok(hipMemcpy(h->d_csc_ptr, hb.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
          (E == 0 || (hip_ok(hipMemcpy(h->d_delay_csc, hb.delay_csc.data(), E, H2D), "copy csc delays") &&
                      hip_ok(hipMemcpy(h->d_pos, hb.pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
-                     hip_ok(hipMemcpy(h->d_perm, hb.perm.data(), sizeof(uint16_t) * E, H2D), "copy perm")));
+                     hip_ok(hipMemcpy(h->d_hpos, hb.hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots")));
     if (!ok) return fail();
   }
   // history pre-fill (init_buffer_kernel, leviathan_kernel.cu:371-372)
@@ -937,15 +927,11 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
 
 int leviathan_set_schedule(LeviathanHandle* h, int schedule, int ept) {
   if (!check_handle(h)) return -1;
-  int idx;
-  if (schedule == LEV_SCHEDULE_BINNED) {
-    idx = kPlanBinned;
-  } else if (schedule == LEV_SCHEDULE_DIRECT && (ept == 4 || ept == 8 || ept == 16)) {
-    idx = plan_of_ept(ept);
-  } else {
-    set_err("schedule must be DIRECT (edges_per_thread 4, 8 or 16) or BINNED");
+  if ((schedule != LEV_SCHEDULE_DIRECT && schedule != LEV_SCHEDULE_BINNED) || (ept != 4 && ept != 8 && ept != 16)) {
+    set_err("schedule must be DIRECT or BINNED with edges_per_thread 4, 8 or 16");
     return -1;
   }
+  const int idx = (schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : 0) + plan_of_ept(ept);
   if (!h->plans[idx].built) {
     set_err("that schedule was not built for this handle (create it with LEV_SCHEDULE_AUTO)");
     return -1;
@@ -962,7 +948,7 @@ int leviathan_set_schedule(LeviathanHandle* h, int schedule, int ept) {
 int leviathan_get_schedule(LeviathanHandle* h, int* schedule, int* ept) {
   if (!check_handle(h)) return -1;
   if (schedule) *schedule = h->plans[h->active].kind;
-  if (ept) *ept = h->plans[h->active].kind == LEV_SCHEDULE_DIRECT ? h->plans[h->active].ept : 0;
+  if (ept) *ept = h->plans[h->active].ept;
   return 0;
 }
 

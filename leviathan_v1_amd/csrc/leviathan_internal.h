@@ -52,6 +52,8 @@ struct StepArgs {
   const int64_t* step;    // global step counter (device)
   const float* drive_b;   // NULL: no input drive
   const float* drive_u;   // [kDriveSteps] ring, u_t at t % kDriveSteps
+  const void* hbuf;       // binned schedule: delayed phases written by the scatter kernel
+  const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
 
@@ -68,8 +70,6 @@ struct FinalizeArgs {
 
 // Two-phase (binned) step: see leviathan_binned.hip.
 constexpr int kScatterEpt = 8;      // phase 1: out-edges per lane in a source block
-constexpr int kBucketSubEpt = 8;    // phase 2: edges per lane in a row sub-block
-constexpr int kMaxBucketLds = 48 * 1024;
 
 struct ScatterArgs {
   const int32_t* csc_ptr;   // [N+1] out-edge offsets per source
@@ -85,16 +85,10 @@ struct ScatterArgs {
   const int64_t* step;
 };
 
-struct BucketArgs {
-  StepArgs s;               // s.blocks = buckets {row_begin, row_end, net, flags}
-  const void* hbuf;
-  const uint16_t* perm;     // [E] CSR edge -> position inside its bucket
-  const int2* sb_index;     // per bucket: {first bound, number of sub-blocks}
-  const int32_t* sb_bounds; // row boundaries of the sub-blocks
-};
-
 // Launchers (leviathan_kernels.hip).  Each returns the launch status.
-hipError_t launch_step(int history, int layout, int ept, const StepArgs& a, hipStream_t s);
+// gather: 0 = delayed phases read from the history ring (direct schedule),
+//         1 = read from the scatter kernel's bucketed array (binned schedule)
+hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
 hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, int32_t n_slots,
                             const float* theta, int32_t n, hipStream_t s);
@@ -102,6 +96,5 @@ hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t p
                             float* out, int32_t n, hipStream_t s);
 // (leviathan_binned.hip)
 hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s);
-hipError_t launch_bucket(int history, int layout, const BucketArgs& a, size_t lds_bytes, hipStream_t s);
 
 }  // namespace lev

@@ -32,7 +32,10 @@
 
 namespace lev {
 
-template <class H, int LAYOUT, int EPT>
+// GATHER = 0: the delayed phase of edge e is read from the history ring at
+// (col[e], t - delay[e]).  GATHER = 1: it was written by the scatter kernel
+// to hbuf[hpos[e]] (binned schedule); col and delay are not read.
+template <class H, int LAYOUT, int EPT, int GATHER>
 __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   constexpr int CAP = kBlock * EPT;
   __shared__ float s_theta[kBlock];
@@ -56,6 +59,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   const int32_t nnz = e1 - e0;
   const int32_t nrows = r1 - r0;
   const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
+  const H* __restrict__ hbuf = reinterpret_cast<const H*>(a.hbuf);
   const int64_t pitch = a.ring_pitch;
   float u_t = 0.0f;
   if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
@@ -84,8 +88,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) {
-        jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
-        dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+        if constexpr (GATHER == 0) {
+          jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
+          dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+        } else {
+          jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+        }
         ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
       }
     }
@@ -94,9 +102,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) {
-        int32_t slot = cur - dd[k];
-        slot += (slot < 0) ? S : 0;
-        hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+        if constexpr (GATHER == 0) {
+          int32_t slot = cur - dd[k];
+          slot += (slot < 0) ? S : 0;
+          hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+        } else {
+          hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
+        }
       }
     }
 #pragma unroll
@@ -148,8 +160,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int k = 0; k < 4; ++k) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
-          jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
-          dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+          if constexpr (GATHER == 0) {
+            jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
+            dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+          } else {
+            jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+          }
           ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
         }
       }
@@ -157,9 +173,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int k = 0; k < 4; ++k) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
-          int32_t slot = cur - dd[k];
-          slot += (slot < 0) ? S : 0;
-          hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+          if constexpr (GATHER == 0) {
+            int32_t slot = cur - dd[k];
+            slot += (slot < 0) ? S : 0;
+            hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
+          } else {
+            hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
+          }
         }
       }
 #pragma unroll
@@ -305,18 +325,18 @@ __global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, flo
 // launchers
 // ---------------------------------------------------------------------------
 
-template <class H, int LAYOUT>
+template <class H, int LAYOUT, int GATHER>
 static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
   const dim3 grid(a.num_blocks), block(kBlock);
   switch (ept) {
     case 4:
-      step_kernel<H, LAYOUT, 4><<<grid, block, 0, s>>>(a);
+      step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
       break;
     case 8:
-      step_kernel<H, LAYOUT, 8><<<grid, block, 0, s>>>(a);
+      step_kernel<H, LAYOUT, 8, GATHER><<<grid, block, 0, s>>>(a);
       break;
     case 16:
-      step_kernel<H, LAYOUT, 16><<<grid, block, 0, s>>>(a);
+      step_kernel<H, LAYOUT, 16, GATHER><<<grid, block, 0, s>>>(a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -324,12 +344,17 @@ static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_step(int history, int layout, int ept, const StepArgs& a, hipStream_t s) {
+template <class H, int LAYOUT>
+static hipError_t launch_step_g(int ept, int gather, const StepArgs& a, hipStream_t s) {
+  return gather ? launch_step_t<H, LAYOUT, 1>(ept, a, s) : launch_step_t<H, LAYOUT, 0>(ept, a, s);
+}
+
+hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
   if (a.num_blocks <= 0) return hipSuccess;
   if (history == 0) {
-    return layout == 0 ? launch_step_t<float, 0>(ept, a, s) : launch_step_t<float, 1>(ept, a, s);
+    return layout == 0 ? launch_step_g<float, 0>(ept, gather, a, s) : launch_step_g<float, 1>(ept, gather, a, s);
   }
-  return layout == 0 ? launch_step_t<uint16_t, 0>(ept, a, s) : launch_step_t<uint16_t, 1>(ept, a, s);
+  return layout == 0 ? launch_step_g<uint16_t, 0>(ept, gather, a, s) : launch_step_g<uint16_t, 1>(ept, gather, a, s);
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {

@@ -15,7 +15,7 @@ inline int deg_of(const int* row_ptr, int r) { return row_ptr[r + 1] - row_ptr[r
 template <class F>
 void parallel_for(int64_t n, F&& f) {
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096) nt = 1;
+  if ((int64_t)nt > n) nt = (unsigned)std::max<int64_t>(n, 1);
   std::vector<std::thread> th;
   const int64_t chunk = (n + nt - 1) / nt;
   for (unsigned t = 0; t < nt; ++t) {
@@ -69,88 +69,38 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
 
 bool build_binned_plan(int N, const int* row_ptr, const int* col, const uint8_t* delays, int n_nets,
                        const int* offs, int bucket_edges, HostBinned& out, std::string& err) {
-  const int cap_sub = kBlock * kBucketSubEpt;
   const int64_t E = row_ptr[N];
-  if (bucket_edges < cap_sub || bucket_edges > 65536) {
-    err = "bucket_edges must be in [" + std::to_string(cap_sub) + ", 65536]";
+  if (bucket_edges < 1024) {
+    err = "bucket_edges must be at least 1024";
     return false;
   }
-  // 1. buckets and their row sub-blocks
-  HostPlan& bk = out.buckets;
-  bk.items.clear();
-  bk.ranges.assign(n_nets, make_int2(0, 0));
-  out.sb_index.clear();
-  out.sb_bounds.clear();
-  out.max_bucket_edges = 0;
-  for (int net = 0; net < n_nets; ++net) {
-    const int lo = offs[net], hi = offs[net + 1];
-    std::vector<int4> longs, shorts;
-    std::vector<std::vector<int32_t>> shorts_b;
-    int r = lo;
-    while (r < hi) {
-      if (deg_of(row_ptr, r) > cap_sub) {
-        longs.push_back(make_int4(r, r + 1, net, kBlockLongRow));
-        ++r;
-        continue;
-      }
-      const int start = r;
-      int nnz = 0;
-      while (r < hi) {
-        const int d = deg_of(row_ptr, r);
-        if (d > cap_sub || nnz + d > bucket_edges) break;
-        nnz += d;
+  (void)n_nets;
+  (void)offs;
+  // 1. buckets over rows
+  std::vector<int32_t> bstart{0};
+  {
+    int r = 0;
+    while (r < N) {
+      int64_t nnz = deg_of(row_ptr, r);
+      ++r;
+      while (r < N && nnz + deg_of(row_ptr, r) <= bucket_edges) {
+        nnz += deg_of(row_ptr, r);
         ++r;
       }
-      out.max_bucket_edges = std::max(out.max_bucket_edges, nnz);
-      std::vector<int32_t> b{start};
-      for (int s = start; s < r;) {
-        int t = s, sn = 0;
-        while (t < r && t - s < kBlock && sn + deg_of(row_ptr, t) <= cap_sub) {
-          sn += deg_of(row_ptr, t);
-          ++t;
-        }
-        b.push_back(t);
-        s = t;
-      }
-      shorts.push_back(make_int4(start, r, net, 0));
-      shorts_b.push_back(std::move(b));
+      bstart.push_back(r);
     }
-    std::stable_sort(longs.begin(), longs.end(),
-                     [&](const int4& a, const int4& b) { return deg_of(row_ptr, a.x) > deg_of(row_ptr, b.x); });
-    bk.ranges[net].x = (int)bk.items.size();
-    for (const int4& it : longs) {
-      bk.items.push_back(it);
-      out.sb_index.push_back(make_int2((int)out.sb_bounds.size(), 0));
-    }
-    for (size_t q = 0; q < shorts.size(); ++q) {
-      bk.items.push_back(shorts[q]);
-      out.sb_index.push_back(make_int2((int)out.sb_bounds.size(), (int)shorts_b[q].size() - 1));
-      out.sb_bounds.insert(out.sb_bounds.end(), shorts_b[q].begin(), shorts_b[q].end());
-    }
-    bk.ranges[net].y = (int)bk.items.size();
   }
+  out.n_buckets = (int32_t)bstart.size() - 1;
 
-  // 2. position of every CSR edge inside its bucket: buckets are ordered by
-  //    (source, CSR index); single-row buckets keep CSR order (= source order).
-  std::vector<uint32_t> pos_of_e((size_t)E);
-  out.perm.assign((size_t)E, 0);
-  const int64_t nb = (int64_t)bk.items.size();
-  parallel_for(nb, [&](int64_t q) {
-    const int4 it = bk.items[q];
-    const int32_t e0 = row_ptr[it.x], e1 = row_ptr[it.y];
+  // 2. slot of every CSR edge: bucket start + rank by (source, CSR index)
+  out.hpos.assign((size_t)E, 0);
+  parallel_for(out.n_buckets, [&](int64_t q) {
+    const int32_t e0 = row_ptr[bstart[q]], e1 = row_ptr[bstart[q + 1]];
     const int32_t n = e1 - e0;
-    if (it.w & kBlockLongRow) {
-      for (int32_t l = 0; l < n; ++l) pos_of_e[e0 + l] = (uint32_t)(e0 + l);
-      return;
-    }
     std::vector<uint64_t> key((size_t)n);
     for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)(uint32_t)col[e0 + l] << 32) | (uint32_t)l;
     std::sort(key.begin(), key.end());
-    for (int32_t k = 0; k < n; ++k) {
-      const uint32_t l = (uint32_t)(key[k] & 0xffffffffu);
-      out.perm[e0 + l] = (uint16_t)k;
-      pos_of_e[e0 + l] = (uint32_t)(e0 + k);
-    }
+    for (int32_t k = 0; k < n; ++k) out.hpos[e0 + (uint32_t)(key[k] & 0xffffffffu)] = (uint32_t)(e0 + k);
   });
 
   // 3. CSC (by source), destinations ascending within a source
@@ -162,15 +112,15 @@ bool build_binned_plan(int N, const int* row_ptr, const int* col, const uint8_t*
   out.delay_csc.assign((size_t)E, 0);
   for (int64_t e = 0; e < E; ++e) {
     const int32_t c = cur[col[e]]++;
-    out.pos[c] = pos_of_e[e];
+    out.pos[c] = out.hpos[e];
     out.delay_csc[c] = delays[e];
   }
-  // every h slot is written exactly once per step: pos is a permutation of [0, E)
+  // every slot is written exactly once per step: pos is a permutation of [0, E)
   {
     std::vector<uint8_t> seen((size_t)E, 0);
     for (int64_t c = 0; c < E; ++c) {
       if (out.pos[c] >= (uint64_t)E || seen[out.pos[c]]) {
-        err = "internal error: bucket positions are not a permutation";
+        err = "internal error: bucket slots are not a permutation";
         return false;
       }
       seen[out.pos[c]] = 1;

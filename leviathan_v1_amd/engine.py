@@ -247,10 +247,10 @@ class LeviathanEngine:
     def autotune(self, dt: float, steps: int = 10) -> dict:
         """Times every built schedule from the current state (which is restored) and keeps the fastest.
         Returns ms per step of each candidate."""
-        out = np.zeros(4, dtype=np.float64)
+        out = np.zeros(6, dtype=np.float64)
         if _lib.lib().leviathan_autotune(self._h, float(dt), int(steps), _p(out)) != 0:
             _lib.check("autotune")
-        names = ["direct/4", "direct/8", "direct/16", "binned"]
+        names = ["direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"]
         return {n: float(v) for n, v in zip(names, out) if v == v}
 
     def close(self) -> None:

@@ -348,7 +348,7 @@ def test_binned_equals_direct(ba200k, history, layout):
     say("binned vs direct", history, layout)
     d = make(ba200k, history=history, layout=layout)
     b = make(ba200k, history=history, layout=layout, schedule="binned")
-    assert b.get_schedule() == ("binned", 0)
+    assert b.get_schedule() == ("binned", 8)
     rd = d.step_n(DT, 40)
     rb = b.step_n(DT, 40)
     assert np.abs(rd - rb).max() < 1e-6
@@ -386,19 +386,19 @@ def test_autotune_restores_state_and_switches(ba200k):
     ref.step_n(DT, 5)
     times = a.autotune(DT, steps=3)
     say("autotune ms/step", times)
-    assert set(times) == {"direct/4", "direct/8", "direct/16", "binned"}
+    assert set(times) == {"direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"}
     assert a.step_count == 5
     assert np.array_equal(a.get_theta(), ref.get_theta())
     assert np.array_equal(a.get_weights(), ref.get_weights())
     best = min(times, key=times.get)
     sched, ept = a.get_schedule()
-    assert (sched == "binned") if best == "binned" else (sched, ept) == ("direct", int(best.split("/")[1]))
+    assert f"{sched}/{ept}" == best
     ra = a.step_n(DT, 20)
     rr = ref.step_n(DT, 20)
     assert np.abs(ra - rr).max() < 1e-6
     assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
-    for s, e in [("direct", 4), ("binned", 0), ("direct", 16)]:
-        a.set_schedule(s, e or 8)
+    for s, e in [("direct", 4), ("binned", 16), ("direct", 16), ("binned", 4)]:
+        a.set_schedule(s, e)
         a.step_n(DT, 3)
         ref.step_n(DT, 3)
     assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
@@ -426,3 +426,24 @@ def test_binned_ensemble_equals_separate():
         r = e.step_n(DT, 60)
         assert np.abs(r - r_ens[:, k]).max() < 1e-6, k
         assert theta_rel_err(e.get_theta(), th[offs[k]:offs[k + 1]]) < 1e-6, k
+
+
+@pytest.mark.parametrize("history,schedule", [("f32", "direct"), ("u16", "direct"), ("u16", "binned")])
+def test_mid_size_long_run_r_statistics(history, schedule):
+    """WS N=100,000 k=20 p=0.1 with plasticity and homeostasis, 10,000 steps: windowed
+    (500-step) mean r and r variance within 5e-3 of the FP64 oracle's run (golden fixture)."""
+    from leviathan_v1_amd import workloads
+    from tests.golden.make_golden import digest
+
+    say("ws100k 10k steps", history, schedule)
+    gold = np.load(os.path.join(HERE, "golden", "ws100k_fp64.npz"))
+    wl = workloads.mid_plain(0)
+    assert str(gold["digest"]) == digest(wl), "workload generator changed"
+    eng = make(wl, history=history, schedule=schedule)
+    r = eng.step_n(DT, 10_000)
+    m, v = windowed(r, 1000, 500)
+    mr, vr = windowed(gold["r"], 1000, 500)
+    say("max |mean r diff|", float(np.abs(m - mr).max()), "max |var diff|", float(np.abs(v - vr).max()),
+        "g", eng.get_gain(), float(gold["g_final"]))
+    assert np.abs(m - mr).max() < 5e-3
+    assert np.abs(v - vr).max() < 5e-3
