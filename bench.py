@@ -149,7 +149,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble"])
+    ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble", "reservoir"])
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
     ap.add_argument("--layout", default="node", choices=["node", "slot"])
     ap.add_argument("--ept", type=int, default=8)
@@ -168,6 +168,14 @@ def main():
         from leviathan_v1_amd import ensemble_bench
 
         print(json.dumps(ensemble_bench.run(args.steps, args.warmup)))
+        return
+    if args.workload == "reservoir":
+        from leviathan_v1_amd import reservoir_bench
+
+        out = reservoir_bench.run(steps=max(args.steps, 2000), oracle_check=not args.no_cpu)
+        print(json.dumps({"metric": "reservoir readout NRMSE (u(t+10)), BF16 MFMA Gram + FP64 refinement",
+                          "value": out["nrmse_test"], "unit": "NRMSE", "n_gpus": 1, "steps": max(args.steps, 2000),
+                          "higher_is_better": False, "data": "synthetic", **out}))
         return
 
     if args.copy_calibration:

@@ -184,6 +184,46 @@ int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_
 int leviathan_set_drive(LeviathanHandle* handle, const float* h_drive_b);
 int leviathan_step_n_driven(LeviathanHandle* handle, float dt, int n, const float* h_u, float* r_out);
 
+/* ---------------------------------------------------------------------------
+ * Reservoir readout support (no counterpart in the reference:
+ * python/reservoir.py:45-60 copies every phase to the host each step and
+ * python/reservoir.py:81-82 fits sklearn's Ridge on the CPU).  Features are
+ * cos(theta) and sin(theta) of n_nodes chosen nodes, captured on the device
+ * after every step into a row-major matrix X[rows][2 * n_nodes] (FP32,
+ * cos block first).
+ * ------------------------------------------------------------------------- */
+
+/* Chooses the feature nodes and reserves max_rows rows (device memory);
+ * clears the captured rows.  Returns 0 or -1. */
+int leviathan_features_setup(LeviathanHandle* handle, const int* nodes, int n_nodes, int max_rows);
+
+/* n steps that each append one feature row.  u (may be NULL) is the drive
+ * input of each step (leviathan_set_drive must have been called when it is
+ * given).  Returns 0 or -1; fails if max_rows would be exceeded. */
+int leviathan_step_n_features(LeviathanHandle* handle, float dt, int n, const float* u, float* r_out);
+
+/* Number of captured rows. */
+int64_t leviathan_features_rows(LeviathanHandle* handle);
+
+/* Copies rows [row0, row0 + nrows) to out[nrows][2 * n_nodes]. */
+int leviathan_features_get(LeviathanHandle* handle, int64_t row0, int64_t nrows, float* out);
+
+/* out[f] = sum_t weights[t] * X[row0 + t][f] over nrows rows, FP64
+ * accumulation (weights NULL: all ones).  Column sums, means and X^T y. */
+int leviathan_features_colsum(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* weights,
+                              double* out);
+
+/* out[t] = sum_f (X[row0 + t][f] - mean[f]) * w[f], FP64 accumulation. */
+int leviathan_features_matvec(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean,
+                              const double* w, double* out);
+
+/* Gram matrix of the centred features over rows [row0, row0 + nrows):
+ * gram_out[F][F] = Xc^T Xc with Xc = X - mean rounded to BF16, multiplied
+ * on the matrix cores with FP32 accumulation (F = 2 * n_nodes).  ms_out
+ * (may be NULL) receives the Gram kernel's duration in milliseconds. */
+int leviathan_features_gram(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean,
+                            float* gram_out, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,13 @@ SIGNATURES = {
     "leviathan_get_schedule": (c_int, [vp, vp, vp]),
     "leviathan_autotune": (c_int, [vp, c_float, c_int, vp]),
     "leviathan_step_n_driven": (c_int, [vp, c_float, c_int, vp, vp]),
+    "leviathan_features_setup": (c_int, [vp, vp, c_int, c_int]),
+    "leviathan_step_n_features": (c_int, [vp, c_float, c_int, vp, vp]),
+    "leviathan_features_rows": (c_int64, [vp]),
+    "leviathan_features_get": (c_int, [vp, c_int64, c_int64, vp]),
+    "leviathan_features_colsum": (c_int, [vp, c_int64, c_int64, vp, vp]),
+    "leviathan_features_matvec": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
+    "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
 }
 
 _lib = None

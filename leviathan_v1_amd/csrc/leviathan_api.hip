@@ -47,6 +47,7 @@ void set_err(const char* fmt, ...) {
 struct GraphEntry {
   uint32_t dt_bits;
   int chunk;
+  bool capture;
   hipGraphExec_t exec;
 };
 
@@ -111,6 +112,16 @@ struct LeviathanHandle {
   float* d_drive_u = nullptr;
   bool drive_on = false;
 
+  // reservoir features
+  int32_t* d_feat_nodes = nullptr;
+  int32_t feat_n = 0;
+  int64_t feat_max_rows = 0;
+  int64_t feat_rows = 0;
+  float* d_X = nullptr;
+  int64_t* d_feat_base = nullptr;
+  double* d_cs_partial = nullptr;  // kColsumChunks * 2 * feat_n
+  bool capturing = false;
+
   float* h_pinned = nullptr;    // kRHistSteps * n_nets floats
   hipStream_t stream = nullptr;
   std::vector<NetCtrl> h_ctrl;  // host copy of the static controller fields
@@ -145,7 +156,8 @@ void destroy(LeviathanHandle* h) {
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
-                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_sources};
+                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_sources,  h->d_feat_nodes,
+                  h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
@@ -247,10 +259,14 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr) 
   return e;
 }
 
-// Enqueue one step on the handle's stream.
+// Enqueue one step on the handle's stream (plus the feature capture while
+// leviathan_step_n_features runs).
 bool enqueue_step(LeviathanHandle* h, float dt) {
   LEV_HIP(enqueue_main(h, dt));
   LEV_HIP(lev::launch_finalize(fin_args(h, dt, h->plans[h->active]), h->stream));
+  if (h->capturing)
+    LEV_HIP(lev::launch_capture(h->d_theta, h->d_feat_nodes, h->feat_n, h->d_X, h->feat_max_rows, h->d_step,
+                                h->d_feat_base, h->stream));
   return true;
 }
 
@@ -258,7 +274,7 @@ bool get_graph(LeviathanHandle* h, float dt, int chunk, hipGraphExec_t* out) {
   uint32_t bits;
   std::memcpy(&bits, &dt, 4);
   for (auto& g : h->graphs)
-    if (g.dt_bits == bits && g.chunk == chunk) {
+    if (g.dt_bits == bits && g.chunk == chunk && g.capture == h->capturing) {
       *out = g.exec;
       return true;
     }
@@ -283,7 +299,7 @@ bool get_graph(LeviathanHandle* h, float dt, int chunk, hipGraphExec_t* out) {
     return false;
   }
   if (h->graphs.size() >= 8) drop_graphs(h);
-  h->graphs.push_back({bits, chunk, exec});
+  h->graphs.push_back({bits, chunk, h->capturing, exec});
   *out = exec;
   return true;
 }
@@ -1022,6 +1038,204 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
   h->active = ok ? best_idx : active0;
   if (!ok) {
     set_err("autotune: a HIP call failed");
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_setup(LeviathanHandle* h, const int* nodes, int n_nodes, int max_rows) {
+  if (!check_handle(h)) return -1;
+  if (!nodes || n_nodes <= 0 || max_rows <= 0) {
+    set_err("need feature nodes and max_rows > 0");
+    return -1;
+  }
+  for (int f = 0; f < n_nodes; ++f)
+    if (nodes[f] < 0 || nodes[f] >= h->N) {
+      set_err("feature node %d out of range", nodes[f]);
+      return -1;
+    }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_err("hipStreamSynchronize failed");
+    return -1;
+  }
+  drop_graphs(h);  // captured graphs hold the old feature buffers
+  void* old[] = {h->d_feat_nodes, h->d_X, h->d_feat_base, h->d_cs_partial};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  h->d_feat_nodes = nullptr;
+  h->d_X = nullptr;
+  h->d_feat_base = nullptr;
+  h->d_cs_partial = nullptr;
+  h->feat_n = n_nodes;
+  h->feat_max_rows = max_rows;
+  h->feat_rows = 0;
+  const size_t F = 2 * (size_t)n_nodes;
+  if (!dalloc(h, &h->d_feat_nodes, (size_t)n_nodes) || !dalloc(h, &h->d_X, (size_t)max_rows * F) ||
+      !dalloc(h, &h->d_feat_base, 1) || !dalloc(h, &h->d_cs_partial, (size_t)lev::kColsumChunks * F)) {
+    h->feat_n = 0;
+    return -1;
+  }
+  if (hipMemcpy(h->d_feat_nodes, nodes, sizeof(int) * n_nodes, hipMemcpyHostToDevice) != hipSuccess) {
+    set_err("copy feature nodes failed");
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_step_n_features(LeviathanHandle* h, float dt, int n, const float* u, float* r_out) {
+  if (!check_handle(h)) return -1;
+  if (h->feat_n == 0) {
+    set_err("call leviathan_features_setup first");
+    return -1;
+  }
+  if (n < 0 || h->feat_rows + n > h->feat_max_rows) {
+    set_err("feature rows would exceed max_rows (%lld + %d > %lld)", (long long)h->feat_rows, n,
+            (long long)h->feat_max_rows);
+    return -1;
+  }
+  if (u && !h->drive_on) {
+    set_err("an input sequence needs a drive: call leviathan_set_drive first");
+    return -1;
+  }
+  // rows are numbered from the step count at which capture of this batch starts
+  const int64_t base = h->step - h->feat_rows;
+  if (hipMemcpy(h->d_feat_base, &base, sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+    set_err("copy feature base failed");
+    return -1;
+  }
+  h->capturing = true;
+  const int seg_max = std::min(lev::kRHistSteps, lev::kDriveSteps);
+  std::vector<float> zeros;
+  bool ok = true;
+  for (int done = 0; done < n && ok;) {
+    const int seg = std::min(n - done, seg_max);
+    if (h->drive_on) {
+      if (!u) zeros.assign(seg, 0.0f);
+      ok = upload_drive_u(h, u ? u + done : zeros.data(), seg);
+    }
+    ok = ok && run_segment(h, dt, seg, r_out ? r_out + (size_t)done * h->n_nets : nullptr);
+    done += seg;
+    if (ok) h->feat_rows += seg;
+  }
+  h->capturing = false;
+  return ok ? 0 : -1;
+}
+
+int64_t leviathan_features_rows(LeviathanHandle* h) { return h ? h->feat_rows : 0; }
+
+namespace {
+bool feat_range(LeviathanHandle* h, int64_t row0, int64_t nrows) {
+  if (h->feat_n == 0) {
+    set_err("no features: call leviathan_features_setup first");
+    return false;
+  }
+  if (row0 < 0 || nrows <= 0 || row0 + nrows > h->feat_rows) {
+    set_err("rows [%lld, %lld) outside the %lld captured rows", (long long)row0, (long long)(row0 + nrows),
+            (long long)h->feat_rows);
+    return false;
+  }
+  return true;
+}
+}  // namespace
+
+int leviathan_features_get(LeviathanHandle* h, int64_t row0, int64_t nrows, float* out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  const size_t F = 2 * (size_t)h->feat_n;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(out, h->d_X + (size_t)row0 * F, sizeof(float) * F * (size_t)nrows, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("features_get: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_colsum(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* weights, double* out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  const int32_t F = 2 * h->feat_n;
+  double *d_w = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc((void**)&d_out, sizeof(double) * F);
+  if (e == hipSuccess && weights) e = hipMalloc((void**)&d_w, sizeof(double) * nrows);
+  if (e == hipSuccess && weights) e = hipMemcpy(d_w, weights, sizeof(double) * nrows, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = lev::launch_colsum(h->d_X, F, row0, nrows, d_w, F, h->d_cs_partial, lev::kColsumChunks, d_out, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(double) * F, hipMemcpyDeviceToHost);
+  if (d_w) (void)hipFree(d_w);
+  if (d_out) (void)hipFree(d_out);
+  if (e != hipSuccess) {
+    set_err("features_colsum: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_matvec(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, const double* w,
+                              double* out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  if (!mean || !w || !out) {
+    set_err("null array");
+    return -1;
+  }
+  const int32_t F = 2 * h->feat_n;
+  double *d_mean = nullptr, *d_w = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc((void**)&d_mean, sizeof(double) * F);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_w, sizeof(double) * F);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_out, sizeof(double) * nrows);
+  if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_w, w, sizeof(double) * F, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = lev::launch_matvec(h->d_X, F, row0, nrows, d_mean, d_w, F, d_out, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(double) * nrows, hipMemcpyDeviceToHost);
+  void* ptrs[] = {d_mean, d_w, d_out};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) {
+    set_err("features_matvec: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_gram(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, float* gram_out,
+                            double* ms_out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  if (!mean || !gram_out) {
+    set_err("null array");
+    return -1;
+  }
+  const int32_t F = 2 * h->feat_n;
+  const int32_t Fp = (F + 127) / 128 * 128;
+  const int64_t Tp = (nrows + 63) / 64 * 64;
+  double* d_mean = nullptr;
+  uint16_t* d_xct = nullptr;
+  float* d_g = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&d_mean, sizeof(double) * F);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_xct, sizeof(uint16_t) * (size_t)Fp * Tp);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_g, sizeof(float) * (size_t)F * F);
+  if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = lev::launch_center_bf16(h->d_X, F, row0, nrows, d_mean, F, Fp, d_xct, Tp, h->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+  if (e == hipSuccess) e = lev::launch_gram(d_xct, Tp, Fp, F, d_g, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && ms_out) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, e0, e1);
+    *ms_out = ms;
+  }
+  if (e == hipSuccess) e = hipMemcpy(gram_out, d_g, sizeof(float) * (size_t)F * F, hipMemcpyDeviceToHost);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  void* ptrs[] = {d_mean, d_xct, d_g};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) {
+    set_err("features_gram: %s", hipGetErrorString(e));
     return -1;
   }
   return 0;

@@ -96,5 +96,16 @@ hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t p
                             float* out, int32_t n, hipStream_t s);
 // (leviathan_binned.hip)
 hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s);
+// (reservoir.hip)
+hipError_t launch_capture(const float* theta, const int32_t* nodes, int32_t n_nodes, float* X, int64_t max_rows,
+                          const int64_t* step, const int64_t* base, hipStream_t s);
+hipError_t launch_colsum(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* w, int32_t F,
+                         double* partial, int32_t chunks, double* out, hipStream_t s);
+hipError_t launch_matvec(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
+                         const double* w, int32_t F, double* out, hipStream_t s);
+hipError_t launch_center_bf16(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
+                              int32_t F, int32_t Fp, uint16_t* XcT, int64_t Tp, hipStream_t s);
+hipError_t launch_gram(const uint16_t* XcT, int64_t Tp, int32_t Fp, int32_t F, float* G, hipStream_t s);
+constexpr int kColsumChunks = 128;
 
 }  // namespace lev

@@ -204,6 +204,58 @@ class LeviathanEngine:
             _lib.check("get_history")
         return out
 
+    # ------------------------------------------------------------------ reservoir features
+    def features_setup(self, nodes, max_rows: int) -> None:
+        nodes = _arr(nodes, np.int32, None, "nodes")
+        self.n_feat_nodes = int(nodes.shape[0])
+        if _lib.lib().leviathan_features_setup(self._h, _p(nodes), self.n_feat_nodes, int(max_rows)) != 0:
+            _lib.check("features_setup")
+
+    def step_n_features(self, dt: float, n: int = None, u=None) -> np.ndarray:
+        """n steps (or len(u) driven steps) that each append one feature row."""
+        ua = None if u is None else _arr(u, np.float32, None, "u")
+        n = int(ua.shape[0]) if ua is not None else int(n)
+        out = np.empty((n, self.n_nets), dtype=np.float32)
+        if _lib.lib().leviathan_step_n_features(self._h, float(dt), n, _p(ua), _p(out)) != 0:
+            _lib.check("step_n_features")
+        return out[:, 0] if self.n_nets == 1 else out
+
+    @property
+    def feature_rows(self) -> int:
+        return int(_lib.lib().leviathan_features_rows(self._h))
+
+    def features_get(self, row0: int, nrows: int) -> np.ndarray:
+        out = np.empty((int(nrows), 2 * self.n_feat_nodes), dtype=np.float32)
+        if _lib.lib().leviathan_features_get(self._h, int(row0), int(nrows), _p(out)) != 0:
+            _lib.check("features_get")
+        return out
+
+    def features_colsum(self, row0: int, nrows: int, weights=None) -> np.ndarray:
+        w = None if weights is None else _arr(weights, np.float64, int(nrows), "weights")
+        out = np.empty(2 * self.n_feat_nodes, dtype=np.float64)
+        if _lib.lib().leviathan_features_colsum(self._h, int(row0), int(nrows), _p(w), _p(out)) != 0:
+            _lib.check("features_colsum")
+        return out
+
+    def features_matvec(self, row0: int, nrows: int, mean, w) -> np.ndarray:
+        F = 2 * self.n_feat_nodes
+        mean = _arr(mean, np.float64, F, "mean")
+        w = _arr(w, np.float64, F, "w")
+        out = np.empty(int(nrows), dtype=np.float64)
+        if _lib.lib().leviathan_features_matvec(self._h, int(row0), int(nrows), _p(mean), _p(w), _p(out)) != 0:
+            _lib.check("features_matvec")
+        return out
+
+    def features_gram(self, row0: int, nrows: int, mean):
+        """(G, ms): BF16-input, FP32-accumulated Gram of the centred features, and the Gram kernel's time."""
+        F = 2 * self.n_feat_nodes
+        mean = _arr(mean, np.float64, F, "mean")
+        g = np.empty((F, F), dtype=np.float32)
+        ms = ctypes.c_double(0.0)
+        if _lib.lib().leviathan_features_gram(self._h, int(row0), int(nrows), _p(mean), _p(g), ctypes.byref(ms)) != 0:
+            _lib.check("features_gram")
+        return g, ms.value
+
     def set_gain_net(self, net: int, g: float) -> None:
         if _lib.lib().leviathan_set_gain_net(self._h, int(net), float(g)) != 0:
             _lib.check("set_gain_net")

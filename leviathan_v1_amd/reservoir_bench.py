@@ -1,0 +1,84 @@
+"""Reservoir readout benchmark.
+
+Reservoir: Watts-Strogatz N=50,000 (k=20, p=0.1, delays uniform in [1, 50]),
+omega_i(t) = omega_i + b_i u(t) with b_i ~ U(-1, 1) and u = mean of 5
+sinusoids with seeded periods in [20, 200] steps plus N(0, 0.05^2) noise.
+50,000 driven steps; features cos/sin of 4,096 seeded nodes (8,192 FP32
+features per step).  Ridge readout (with intercept, alpha = 1e-4) predicts
+u(t + 10); the first 80 % of the rows after a 1,000-step washout train it, the
+rest test it.  Reported against an FP64 Gram/solve of the same features on
+the CPU (the oracle).
+"""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+
+from . import workloads
+from .reservoir import LeviathanReservoir, multisine_input, nrmse
+
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def _log(*a):
+    print("[reservoir]", *a, file=sys.stderr, flush=True)
+
+
+def run(N=50_000, steps=50_000, n_nodes=4096, lag=10, alpha=1e-4, washout=1000, seed=0, oracle_check=True):
+    t0 = time.perf_counter()
+    wl = workloads.watts_strogatz_workload(N, 20, 0.1, 50, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    nodes = np.sort(rng.choice(N, n_nodes, replace=False)).astype(np.int32)
+    b = rng.uniform(-1.0, 1.0, N).astype(np.float32)
+    u = multisine_input(steps, seed=seed + 11)
+    res = LeviathanReservoir(wl, nodes, b, max_rows=steps)
+    _log(f"setup {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    res.run(u)
+    run_s = time.perf_counter() - t0
+    _log(f"{steps} driven steps with capture: {run_s:.2f} s")
+
+    rows = steps - washout - lag
+    n_train = int(0.8 * rows)
+    row0 = washout
+    y = u[row0 + lag: row0 + lag + rows].astype(np.float64)
+    fit = res.fit(y[:n_train], row0, alpha=alpha)
+    _log(f"fit: {fit.iterations} refinement iterations, rel residual {fit.rel_residual:.2e}, timings {fit.timings}")
+    pred = res.predict(fit, row0, rows)
+    out = {
+        "rows_train": n_train, "rows_test": rows - n_train,
+        "nrmse_train": nrmse(pred[:n_train], y[:n_train]),
+        "nrmse_test": nrmse(pred[n_train:], y[n_train:]),
+        "refine_iterations": fit.iterations, "rel_residual": fit.rel_residual, "preconditioner_shift": fit.shift,
+        "timings": fit.timings, "run_s": run_s,
+    }
+    F = 2 * n_nodes
+    Fp = (F + 127) // 128 * 128
+    Tp = (n_train + 63) // 64 * 64
+    nt = Fp // 128
+    mfma_flops = 2.0 * Tp * 128 * 128 * nt * (nt + 1) / 2
+    ms = fit.timings["gram_kernel_ms"]
+    out["gram"] = {"flop_full_product": 2.0 * n_train * F * F, "flop_issued": mfma_flops, "kernel_ms": ms,
+                   "tflops_issued": mfma_flops / (ms * 1e-3) / 1e12,
+                   "roofline": {"bound": "mfma", "achieved": mfma_flops / (ms * 1e-3) / 1e12,
+                                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": mfma_flops / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS}}
+    if oracle_check:
+        import oracle
+
+        t0 = time.perf_counter()
+        X = res.features(row0, rows)
+        w64, c64 = oracle.ridge_fp64(X[:n_train], y[:n_train], alpha)
+        pred64 = X.astype(np.float64) @ w64 + c64
+        out["oracle"] = {
+            "nrmse_train": nrmse(pred64[:n_train], y[:n_train]),
+            "nrmse_test": nrmse(pred64[n_train:], y[n_train:]),
+            "solution_rel_err": float(np.linalg.norm(fit.w - w64) / np.linalg.norm(w64)),
+            "prediction_max_abs_diff": float(np.abs(pred - pred64).max()),
+            "seconds": time.perf_counter() - t0,
+        }
+        _log("oracle", out["oracle"])
+    res.close()
+    return out

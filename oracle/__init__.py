@@ -274,3 +274,27 @@ def circ_err(a, b):
     """|a - b| on the circle, in radians."""
     d = np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)
     return np.abs(d - np.round(d / TWO_PI) * TWO_PI)
+
+
+def ridge_fp64(X, y, alpha):
+    """FP64 ridge regression with an intercept (the estimator of sklearn's Ridge with
+    fit_intercept=True, which python/reservoir.py:81-82 uses): centre X and y, solve
+    (Xc^T Xc + alpha I) w = Xc^T yc by Cholesky, intercept = mean(y) - mean(X) w."""
+    import scipy.linalg as sla
+
+    X = np.asarray(X, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    mu = X.mean(axis=0)
+    Xc = X - mu
+    yc = y - y.mean()
+    G = Xc.T @ Xc
+    G[np.diag_indices_from(G)] += alpha
+    w = sla.cho_solve(sla.cho_factor(G, lower=True), Xc.T @ yc)
+    return w, float(y.mean() - mu @ w)
+
+
+def bf16_round(x):
+    """Round float32 values to the nearest BF16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)

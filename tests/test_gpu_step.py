@@ -447,3 +447,27 @@ def test_mid_size_long_run_r_statistics(history, schedule):
         "g", eng.get_gain(), float(gold["g_final"]))
     assert np.abs(m - mr).max() < 5e-3
     assert np.abs(v - vr).max() < 5e-3
+
+
+def test_ensemble_r_of_K_vs_oracle():
+    """Three networks (K = 0.5, 2, 4, controller off) in one batched engine: per-network
+    windowed r statistics over 2,000 steps within 5e-3 of each network's FP64 oracle run."""
+    from leviathan_v1_amd import workloads
+    from leviathan_v1_amd.ensemble import LeviathanEnsemble
+    from oracle import COracle, Params
+
+    say("ensemble r(K) vs oracle")
+    gains = [0.5, 2.0, 4.0]
+    wls = [workloads.watts_strogatz_workload(2000, 20, 0.1, 50, seed=50 + i) for i in range(3)]
+    ens = LeviathanEnsemble(wls, gains, history="u16", schedule="auto")
+    ens.engine.autotune(DT, steps=2)
+    K, r_mean, r_std = ens.phase_sweep(0, 2000)
+    ens2 = LeviathanEnsemble(wls, gains)
+    r_all = ens2.step_n(2000)
+    for k, w in enumerate(wls):
+        o = COracle(*w.engine_args(), params=Params(g0=gains[k], gain_controller=False), dtype=np.float64)
+        r_ref = o.run(DT, 2000)
+        m, v = windowed(r_all[:, k], 500, 500)
+        mr, vr = windowed(r_ref, 500, 500)
+        assert np.abs(m - mr).max() < 5e-3 and np.abs(v - vr).max() < 5e-3, k
+        assert abs(r_mean[k] - r_ref.mean()) < 5e-3, k

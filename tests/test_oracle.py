@@ -268,3 +268,30 @@ def test_fp32_port_tracks_fp64_short_horizon(oracle_lib):
 def test_oracle_rejects_delay_beyond_max():
     with pytest.raises(ValueError):
         tiny("numpy", 2, [(0, 1, 6, 0.5)], [0.0, 0.0], max_delay=5)
+
+
+def test_ridge_fp64_matches_sklearn_and_exact_fit():
+    """The FP64 ridge oracle against sklearn's Ridge(fit_intercept=True) (a library oracle,
+    the estimator python/reservoir.py:81-82 uses), and exact recovery of a noiseless linear map."""
+    from sklearn.linear_model import Ridge
+
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((400, 30))
+    y = X @ rng.standard_normal(30) + 0.7 + 0.1 * rng.standard_normal(400)
+    for alpha in (1e-4, 1.0, 50.0):
+        w, c = oracle.ridge_fp64(X, y, alpha)
+        ref = Ridge(alpha=alpha, fit_intercept=True, solver="cholesky").fit(X, y)
+        assert np.abs(w - ref.coef_).max() < 1e-9
+        assert abs(c - ref.intercept_) < 1e-9
+    wt = rng.standard_normal(30)
+    w, c = oracle.ridge_fp64(X, X @ wt - 2.5, 1e-12)
+    assert np.abs(w - wt).max() < 1e-8 and abs(c + 2.5) < 1e-8
+
+
+def test_bf16_round_known_values():
+    """BF16 rounding: exact values stay, halfway cases go to even, relative error <= 2^-8 (half an ulp)."""
+    x = np.array([1.0, 1.0 + 2**-8, 1.0 + 2**-7, 1.0 + 3 * 2**-8, -2.5, 0.0], np.float32)
+    want = np.array([1.0, 1.0, 1.0 + 2**-7, 1.0 + 2**-6, -2.5, 0.0], np.float32)
+    assert np.array_equal(oracle.bf16_round(x), want)
+    r = np.random.default_rng(0).standard_normal(10000).astype(np.float32)
+    assert (np.abs(oracle.bf16_round(r) - r) / np.abs(r)).max() <= 2**-8
