@@ -133,15 +133,21 @@ def copy_calibration(nbytes: int, reps: int = 3) -> None:
     log(f"copy calibration: {reps} x {nbytes} bytes device-to-device")
 
 
-def load_traffic(workload, history):
+def load_traffic(workload, history, schedule):
+    """HBM bytes per step of the step's kernels, from the rocprofv3 counter runs
+    summarised in profiles/traffic.json (None if that workload was not profiled)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get(f"{workload}:{history}")
+            rec = json.load(f).get(f"{workload}:{history}:{schedule}")
     except (OSError, ValueError):
         return None
+    if not rec:
+        return None
+    ks = {k: v for k, v in rec["kernels"].items() if "step_kernel" in k or "scatter_kernel" in k}
+    return sum(v["fetch_bytes"] + v["write_bytes"] for v in ks.values()) or None
 
 
 def main():
@@ -218,7 +224,8 @@ def main():
         "speedup_u16_vs_fp32": (forms["u16"]["edge_steps_per_s"] / forms["f32"]["edge_steps_per_s"]
                                 if "u16" in forms and "f32" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload, hist),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(args.workload, hist, f"{head['schedule']}/{head['edges_per_thread']}"),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }
