@@ -79,7 +79,7 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         theta = eng.get_theta()
         ok = bool(np.isfinite(theta).all() and np.isfinite(r).all() and 0 <= r.min() and r.max() <= 1)
         return {
-            "history": history, "layout": layout, "schedule": chosen[0], "edges_per_thread": chosen[1],
+            "history": history, "layout": layout, "schedule": eng.schedule_name,
             "autotune_ms_per_step": tuned,
             "seconds": el, "ms_per_step": 1e3 * el / steps,
             "edge_steps_per_s": wl.E * steps / el,
@@ -146,7 +146,8 @@ def load_traffic(workload, history, schedule):
         return None
     if not rec:
         return None
-    ks = {k: v for k, v in rec["kernels"].items() if "step_kernel" in k or "scatter_kernel" in k}
+    ks = {k: v for k, v in rec["kernels"].items()
+          if "step_kernel" in k or "scatter_kernel" in k or "bucket_kernel" in k}
     return sum(v["fetch_bytes"] + v["write_bytes"] for v in ks.values()) or None
 
 
@@ -161,6 +162,8 @@ def main():
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned"])
     ap.add_argument("--bucket-edges", type=int, default=0)
+    ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
+    ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="f32,u16")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -178,7 +181,8 @@ def main():
     if args.workload == "reservoir":
         from leviathan_v1_amd import reservoir_bench
 
-        out = reservoir_bench.run(steps=max(args.steps, 2000), oracle_check=not args.no_cpu)
+        out = reservoir_bench.run(steps=max(args.steps, 2000), oracle_check=not args.no_cpu,
+                                  static=args.static_reservoir, graph_chunk=args.graph_chunk)
         print(json.dumps({"metric": "reservoir readout NRMSE (u(t+10)), BF16 MFMA Gram + FP64 refinement",
                           "value": out["nrmse_test"], "unit": "NRMSE", "n_gpus": 1, "steps": max(args.steps, 2000),
                           "higher_is_better": False, "data": "synthetic", **out}))
@@ -198,7 +202,7 @@ def main():
     alg = algorithmic_bytes(wl.N, wl.E, hist)
     # the step's kernels: the fused step kernel (direct) or scatter + bucket kernels (binned)
     step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"]
-    kernels = "step_kernel" if head["schedule"] == "direct" else "scatter_kernel+bucket_kernel"
+    kernels = "scatter_kernel+bucket_kernel" if head["schedule"] == "binned" else "step_kernel"
     achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -217,7 +221,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded graph and initial state)",
         "config": {"workload": wl.name, "N": wl.N, "E": wl.E, "max_delay": wl.max_delay,
-                   "history": hist, "layout": args.layout, "edges_per_thread": args.ept},
+                   "history": hist, "layout": args.layout, "schedule": head["schedule"]},
         "hbm_gbs": head["hbm_gbs"],
         "plain_fp32": forms.get("f32"),
         "u16": forms.get("u16"),
@@ -225,7 +229,7 @@ def main():
                                 if "u16" in forms and "f32" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic(args.workload, hist, f"{head['schedule']}/{head['edges_per_thread']}"),
+                     "traffic": load_traffic(args.workload, hist, head["schedule"]),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }

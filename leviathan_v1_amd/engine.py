@@ -296,6 +296,11 @@ class LeviathanEngine:
             _lib.check("get_schedule")
         return ({v: k for k, v in _lib.SCHEDULE.items()}[s.value], e.value)
 
+    @property
+    def schedule_name(self) -> str:
+        s, e = self.get_schedule()
+        return f"{s}/{e}"
+
     def autotune(self, dt: float, steps: int = 10) -> dict:
         """Times every built schedule from the current state (which is restored) and keeps the fastest.
         Returns ms per step of each candidate."""

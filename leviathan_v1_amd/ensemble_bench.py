@@ -32,7 +32,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
 
     ens = LeviathanEnsemble(wls, spec.gains, history=history, schedule="auto", graph_chunk=100)
     tuned = ens.engine.autotune(ens.dt, steps=5)
-    chosen = ens.engine.get_schedule()
+    chosen = ens.engine.schedule_name
     _log("autotune", tuned, "->", chosen)
     ens.step_n(warmup)
     t0 = time.perf_counter()
@@ -75,7 +75,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
         "data": "synthetic (seeded graphs and initial states)",
         "config": {"workload": f"{count} Watts-Strogatz networks k=20 p=0.1, sizes log-uniform [2000, 200000], "
                                f"K=linspace(0,4), controller off", "N": N, "E": E, "history": history,
-                   "schedule": f"{chosen[0]}/{chosen[1]}", "graph_chunk": 100},
+                   "schedule": chosen, "graph_chunk": 100},
         "autotune_ms_per_step": tuned,
         "step_kernel_ms": prof["step_kernel_ms"] + prof["scatter_kernel_ms"],
         "sequential_plain": {"value": sequential, "unit": "edge-steps/s", "steps": seq_steps,

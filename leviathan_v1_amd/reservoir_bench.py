@@ -26,14 +26,18 @@ def _log(*a):
     print("[reservoir]", *a, file=sys.stderr, flush=True)
 
 
-def run(N=50_000, steps=50_000, n_nodes=4096, lag=10, alpha=1e-4, washout=1000, seed=0, oracle_check=True):
+def run(N=50_000, steps=50_000, n_nodes=4096, lag=10, alpha=1e-4, washout=1000, seed=0, oracle_check=True,
+        static=False, graph_chunk=100):
+    """static=True freezes the reservoir: no plasticity (eta = lam = 0) and no gain controller, so the
+    readout is trained and tested on a stationary system."""
     t0 = time.perf_counter()
     wl = workloads.watts_strogatz_workload(N, 20, 0.1, 50, seed=seed)
     rng = np.random.Generator(np.random.PCG64(seed + 7))
     nodes = np.sort(rng.choice(N, n_nodes, replace=False)).astype(np.int32)
     b = rng.uniform(-1.0, 1.0, N).astype(np.float32)
     u = multisine_input(steps, seed=seed + 11)
-    res = LeviathanReservoir(wl, nodes, b, max_rows=steps)
+    params = dict(eta=0.0, lam=0.0, gain_controller=False) if static else None
+    res = LeviathanReservoir(wl, nodes, b, max_rows=steps, params=params, graph_chunk=graph_chunk)
     _log(f"setup {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     res.run(u)
@@ -48,6 +52,7 @@ def run(N=50_000, steps=50_000, n_nodes=4096, lag=10, alpha=1e-4, washout=1000, 
     _log(f"fit: {fit.iterations} refinement iterations, rel residual {fit.rel_residual:.2e}, timings {fit.timings}")
     pred = res.predict(fit, row0, rows)
     out = {
+        "static_reservoir": static,
         "rows_train": n_train, "rows_test": rows - n_train,
         "nrmse_train": nrmse(pred[:n_train], y[:n_train]),
         "nrmse_test": nrmse(pred[n_train:], y[n_train:]),

@@ -391,8 +391,7 @@ def test_autotune_restores_state_and_switches(ba200k):
     assert np.array_equal(a.get_theta(), ref.get_theta())
     assert np.array_equal(a.get_weights(), ref.get_weights())
     best = min(times, key=times.get)
-    sched, ept = a.get_schedule()
-    assert f"{sched}/{ept}" == best
+    assert a.schedule_name == best
     ra = a.step_n(DT, 20)
     rr = ref.step_n(DT, 20)
     assert np.abs(ra - rr).max() < 1e-6
@@ -401,7 +400,12 @@ def test_autotune_restores_state_and_switches(ba200k):
         a.set_schedule(s, e)
         a.step_n(DT, 3)
         ref.step_n(DT, 3)
+        assert np.array_equal(a.get_weights(), ref.get_weights()), (s, e)
     assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
+    w = ref.get_weights()
+    w[::7] = 0.25
+    a.reset_weights(w)
+    assert np.array_equal(a.get_weights(), w)
     with pytest.raises(RuntimeError):
         ref.set_schedule("binned")
 
