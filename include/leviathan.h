@@ -218,10 +218,12 @@ int leviathan_features_matvec(LeviathanHandle* handle, int64_t row0, int64_t nro
                               const double* w, double* out);
 
 /* Gram matrix of the centred features over rows [row0, row0 + nrows):
- * gram_out[F][F] = Xc^T Xc with Xc = X - mean rounded to BF16, multiplied
- * on the matrix cores with FP32 accumulation (F = 2 * n_nodes).  ms_out
+ * gram_out[F][F] = Xc^T Xc with Xc = X - mean, on the matrix cores with FP32
+ * accumulation (F = 2 * n_nodes).  terms = 1: Xc rounded to BF16, one BF16
+ * product; terms = 3: Xc split into BF16 hi + lo parts and the Gram formed as
+ * hi hi^T + hi lo^T + lo hi^T (three BF16 products, ~16-bit inputs).  ms_out
  * (may be NULL) receives the Gram kernel's duration in milliseconds. */
-int leviathan_features_gram(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean,
+int leviathan_features_gram(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean, int terms,
                             float* gram_out, double* ms_out);
 
 #ifdef __cplusplus

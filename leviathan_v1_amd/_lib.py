@@ -83,7 +83,7 @@ SIGNATURES = {
     "leviathan_features_get": (c_int, [vp, c_int64, c_int64, vp]),
     "leviathan_features_colsum": (c_int, [vp, c_int64, c_int64, vp, vp]),
     "leviathan_features_matvec": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
-    "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
+    "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, c_int, vp, vp]),
 }
 
 _lib = None

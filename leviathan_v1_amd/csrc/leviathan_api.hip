@@ -1198,29 +1198,33 @@ int leviathan_features_matvec(LeviathanHandle* h, int64_t row0, int64_t nrows, c
   return 0;
 }
 
-int leviathan_features_gram(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, float* gram_out,
-                            double* ms_out) {
+int leviathan_features_gram(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, int terms,
+                            float* gram_out, double* ms_out) {
   if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
-  if (!mean || !gram_out) {
-    set_err("null array");
+  if (!mean || !gram_out || (terms != 1 && terms != 3)) {
+    set_err("need mean and output arrays and terms 1 or 3");
     return -1;
   }
   const int32_t F = 2 * h->feat_n;
-  const int32_t Fp = (F + 127) / 128 * 128;
+  const int32_t Fp = (F + 255) / 256 * 256;  // whole 256-column Gram tiles
   const int64_t Tp = (nrows + 63) / 64 * 64;
   double* d_mean = nullptr;
-  uint16_t* d_xct = nullptr;
+  uint16_t *d_xct = nullptr, *d_lo = nullptr;
   float* d_g = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipMalloc((void**)&d_mean, sizeof(double) * F);
   if (e == hipSuccess) e = hipMalloc((void**)&d_xct, sizeof(uint16_t) * (size_t)Fp * Tp);
+  if (e == hipSuccess && terms == 3) e = hipMalloc((void**)&d_lo, sizeof(uint16_t) * (size_t)Fp * Tp);
   if (e == hipSuccess) e = hipMalloc((void**)&d_g, sizeof(float) * (size_t)F * F);
   if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = lev::launch_center_bf16(h->d_X, F, row0, nrows, d_mean, F, Fp, d_xct, Tp, h->stream);
+  if (e == hipSuccess)
+    e = lev::launch_center_bf16(h->d_X, F, row0, nrows, d_mean, F, Fp, d_xct, Tp, d_lo, h->stream);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-  if (e == hipSuccess) e = lev::launch_gram(d_xct, Tp, Fp, F, d_g, h->stream);
+  if (e == hipSuccess)
+    e = terms == 1 ? lev::launch_gram(d_xct, Tp, Fp, F, d_g, h->stream)
+                   : lev::launch_gram_split(d_xct, d_lo, Tp, Fp, F, d_g, h->stream);
   if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess && ms_out) {
@@ -1231,7 +1235,7 @@ int leviathan_features_gram(LeviathanHandle* h, int64_t row0, int64_t nrows, con
   if (e == hipSuccess) e = hipMemcpy(gram_out, d_g, sizeof(float) * (size_t)F * F, hipMemcpyDeviceToHost);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  void* ptrs[] = {d_mean, d_xct, d_g};
+  void* ptrs[] = {d_mean, d_xct, d_lo, d_g};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (e != hipSuccess) {

@@ -104,8 +104,10 @@ hipError_t launch_colsum(const float* X, int64_t ld, int64_t row0, int64_t nrows
 hipError_t launch_matvec(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
                          const double* w, int32_t F, double* out, hipStream_t s);
 hipError_t launch_center_bf16(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
-                              int32_t F, int32_t Fp, uint16_t* XcT, int64_t Tp, hipStream_t s);
+                              int32_t F, int32_t Fp, uint16_t* XcT, int64_t Tp, uint16_t* XcT_lo, hipStream_t s);
 hipError_t launch_gram(const uint16_t* XcT, int64_t Tp, int32_t Fp, int32_t F, float* G, hipStream_t s);
+hipError_t launch_gram_split(const uint16_t* Hi, const uint16_t* Lo, int64_t Tp, int32_t Fp, int32_t F, float* G,
+                             hipStream_t s);
 constexpr int kColsumChunks = 128;
 
 }  // namespace lev

@@ -9,10 +9,10 @@
 //   colsum_kernel        sum_t w_t X[t][f] in FP64 (means, X^T y, X^T v)
 //   matvec_kernel        sum_f (X[t][f] - mu_f) w_f in FP64 (residuals)
 //   center_bf16_kernel   Xc^T = (X - mu)^T rounded to BF16, feature-major,
-//                        zero-padded to whole 128-feature / 16-sample tiles
+//                        zero-padded to whole 256-feature / 64-sample tiles
 //   gram_kernel          G = Xc^T Xc on the matrix cores: v_mfma_f32_32x32x16_bf16
-//                        (BF16 inputs, FP32 accumulation); one 128x128 tile per
-//                        workgroup, upper-triangle tiles only, mirrored.
+//                        (BF16 inputs, FP32 accumulation); 128x256 tiles on or
+//                        above the diagonal, LDS double buffering, mirrored.
 //
 // The host factorises G + alpha I once in FP64 and refines the solution with
 // residuals b - (Xc^T Xc + alpha I) w computed by the FP64 kernels above, so
@@ -93,7 +93,8 @@ __device__ __forceinline__ uint16_t to_bf16_rne(float x) {
 // the padding (F <= f < Fp, nrows <= t < Tp).  64x64 tiles through LDS.
 __global__ __launch_bounds__(256) void center_bf16_kernel(const float* __restrict__ X, int64_t ld, int64_t row0,
                                                           int64_t nrows, const double* __restrict__ mean,
-                                                          int32_t F, uint16_t* __restrict__ XcT, int64_t Tp) {
+                                                          int32_t F, uint16_t* __restrict__ XcT, int64_t Tp,
+                                                          uint16_t* __restrict__ XcT_lo) {
   __shared__ float tile[64][65];
   const int32_t f0 = blockIdx.x * 64;
   const int64_t t0 = (int64_t)blockIdx.y * 64;
@@ -109,19 +110,22 @@ __global__ __launch_bounds__(256) void center_bf16_kernel(const float* __restric
   for (int r = ty; r < 64; r += 4) {
     const int32_t f = f0 + r;
     const int64_t t = t0 + tx;
-    XcT[(int64_t)f * Tp + t] = to_bf16_rne(tile[tx][r]);
+    const float x = tile[tx][r];
+    const uint16_t hi = to_bf16_rne(x);
+    XcT[(int64_t)f * Tp + t] = hi;
+    // split form: lo = bf16(x - hi), so hi + lo carries ~16 significant bits
+    if (XcT_lo) XcT_lo[(int64_t)f * Tp + t] = to_bf16_rne(x - __uint_as_float((uint32_t)hi << 16));
   }
 }
 
-// G[i][j] = sum_t XcT[i][t] XcT[j][t] for the 128x128 tile (ti, tj), ti <= tj,
-// written to both (i, j) and (j, i).  Four waves, each a 64x64 quarter as
-// 2x2 blocks of v_mfma_f32_32x32x16_bf16.  Operand fragments: lane l holds
-// row (l % 32) of its 32-row block and the 8 consecutive samples
-// k0 + 8 * (l / 32) .. +7; accumulator register v of lane l is element
-// (8 * (v / 4) + 4 * (l / 32) + v % 4, l % 32) of the 32x32 block.
-__global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ XcT, int64_t Tp, int32_t nt,
-                                                   int32_t F, float* __restrict__ G) {
-  // upper-triangle tile enumeration, row by row
+// Split-BF16 Gram: G = Hi Hi^T + Hi Lo^T + Lo Hi^T with X = Hi + Lo, three
+// BF16 products per block on the matrix cores, FP32 accumulation; the
+// relative error of the inputs drops from 2^-9 to ~2^-17.  128x128 tiles on
+// or above the diagonal, four waves of 64x64, fragments read straight from
+// the L2 (the layouts of gram_kernel's comment apply).
+__global__ __launch_bounds__(256) void gram_split_kernel(const uint16_t* __restrict__ Hi,
+                                                         const uint16_t* __restrict__ Lo, int64_t Tp, int32_t nt,
+                                                         int32_t F, float* __restrict__ G) {
   int32_t b = blockIdx.x, ti = 0;
   while (b >= nt - ti) {
     b -= nt - ti;
@@ -133,19 +137,32 @@ __global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ 
   const int32_t jb = tj * 128 + (wave & 1) * 64;
   const int lm = lane & 31;
   const int64_t lk = (lane >> 5) * 8;
-  const bf16x8* a0 = reinterpret_cast<const bf16x8*>(XcT + (int64_t)(ib + lm) * Tp + lk);
-  const bf16x8* a1 = reinterpret_cast<const bf16x8*>(XcT + (int64_t)(ib + 32 + lm) * Tp + lk);
-  const bf16x8* b0 = reinterpret_cast<const bf16x8*>(XcT + (int64_t)(jb + lm) * Tp + lk);
-  const bf16x8* b1 = reinterpret_cast<const bf16x8*>(XcT + (int64_t)(jb + 32 + lm) * Tp + lk);
+  const int64_t oa0 = (int64_t)(ib + lm) * Tp + lk, oa1 = (int64_t)(ib + 32 + lm) * Tp + lk;
+  const int64_t ob0 = (int64_t)(jb + lm) * Tp + lk, ob1 = (int64_t)(jb + 32 + lm) * Tp + lk;
   f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
-  const int64_t steps = Tp / 16;  // Tp is a multiple of 64
-#pragma unroll 4
+  const int64_t steps = Tp / 16;
   for (int64_t s = 0; s < steps; ++s) {
-    const bf16x8 va0 = a0[2 * s], va1 = a1[2 * s], vb0 = b0[2 * s], vb1 = b1[2 * s];
-    c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va0, vb0, c00, 0, 0, 0);
-    c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va0, vb1, c01, 0, 0, 0);
-    c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va1, vb0, c10, 0, 0, 0);
-    c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(va1, vb1, c11, 0, 0, 0);
+    const int64_t k = 16 * s;
+    const bf16x8 ha0 = *reinterpret_cast<const bf16x8*>(Hi + oa0 + k);
+    const bf16x8 ha1 = *reinterpret_cast<const bf16x8*>(Hi + oa1 + k);
+    const bf16x8 hb0 = *reinterpret_cast<const bf16x8*>(Hi + ob0 + k);
+    const bf16x8 hb1 = *reinterpret_cast<const bf16x8*>(Hi + ob1 + k);
+    const bf16x8 la0 = *reinterpret_cast<const bf16x8*>(Lo + oa0 + k);
+    const bf16x8 la1 = *reinterpret_cast<const bf16x8*>(Lo + oa1 + k);
+    const bf16x8 lb0 = *reinterpret_cast<const bf16x8*>(Lo + ob0 + k);
+    const bf16x8 lb1 = *reinterpret_cast<const bf16x8*>(Lo + ob1 + k);
+    c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la0, hb0, c00, 0, 0, 0);
+    c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha0, lb0, c00, 0, 0, 0);
+    c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha0, hb0, c00, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la0, hb1, c01, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha0, lb1, c01, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha0, hb1, c01, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la1, hb0, c10, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha1, lb0, c10, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha1, hb0, c10, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(la1, hb1, c11, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha1, lb1, c11, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ha1, hb1, c11, 0, 0, 0);
   }
   const f32x16 cc[2][2] = {{c00, c01}, {c10, c11}};
 #pragma unroll
@@ -156,9 +173,119 @@ __global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ 
       for (int v = 0; v < 16; ++v) {
         const int32_t i = ib + 32 * p + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
         const int32_t j = jb + 32 * q + lm;
-        if (i < F && j < F) {
+        // (i, j) and (j, i) sum the three products in different orders: keep
+        // i <= j and mirror it, so the result is exactly symmetric
+        if (i < F && j < F && (ti != tj || i <= j)) {
           G[(int64_t)i * F + j] = cc[p][q][v];
-          if (ti != tj) G[(int64_t)j * F + i] = cc[p][q][v];
+          G[(int64_t)j * F + i] = cc[p][q][v];
+        }
+      }
+}
+
+// G = XcT XcT^T on the matrix cores.  Workgroup tile: 128 rows (i) x 256
+// columns (j); tiles entirely below the diagonal are skipped, and inside a
+// tile only i <= j is stored, together with its mirror (j, i).  Four waves,
+// each a 64 x 128 sub-tile = 2 x 4 blocks of v_mfma_f32_32x32x16_bf16 (eight
+// 32x32 FP32 accumulators).  The K dimension (samples) advances 32 at a time
+// through double-buffered LDS: while the matrix cores work on one slice the
+// next slice is already in registers.  LDS rows are padded from 64 to 80
+// bytes so the 16-byte fragment reads of 32 consecutive rows spread over the
+// banks.
+//
+// MFMA operand fragments: lane l holds row (l % 32) of its 32-row block and
+// samples 8 * (l / 32) .. +7 of the 16-sample step; accumulator register v of
+// lane l is element (8 * (v / 4) + 4 * (l / 32) + v % 4, l % 32) of the block.
+constexpr int kGramBM = 128, kGramBN = 256, kGramBK = 32;
+constexpr int kGramPitch = 40;  // bf16 per LDS row: 32 samples + 8 padding
+
+__global__ __launch_bounds__(256) void gram_kernel(const uint16_t* __restrict__ XcT, int64_t Tp, int32_t nI,
+                                                   int32_t nJ, int32_t F, float* __restrict__ G) {
+  __shared__ __align__(16) uint16_t sA[2][kGramBM * kGramPitch];
+  __shared__ __align__(16) uint16_t sB[2][kGramBN * kGramPitch];
+  // tile enumeration: row block I needs column blocks J >= I / 2
+  int32_t b = blockIdx.x, I = 0;
+  while (b >= nJ - I / 2) {
+    b -= nJ - I / 2;
+    ++I;
+  }
+  const int32_t J = I / 2 + b;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int32_t i0 = I * kGramBM, j0 = J * kGramBN;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;  // wave's sub-tile inside the workgroup tile
+
+  // global -> register staging: A slice is 128 rows x 64 B (512 chunks of 16 B),
+  // B slice 256 rows x 64 B (1024 chunks); thread tid takes chunks tid + 256 q.
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 ra[2], rb[4];
+  auto load_slice = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+      ra[q] = *reinterpret_cast<const u32x4*>(XcT + (int64_t)(i0 + row) * Tp + k0 + part * 8);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+      rb[q] = *reinterpret_cast<const u32x4*>(XcT + (int64_t)(j0 + row) * Tp + k0 + part * 8);
+    }
+  };
+  auto store_slice = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+      *reinterpret_cast<u32x4*>(&sA[buf][row * kGramPitch + part * 8]) = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+      *reinterpret_cast<u32x4*>(&sB[buf][row * kGramPitch + part * 8]) = rb[q];
+    }
+  };
+
+  f32x16 acc[2][4];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = f32x16{};
+
+  const int64_t nk = Tp / kGramBK;
+  load_slice(0);
+  store_slice(0);
+  __syncthreads();
+  const int lrow = lane & 31, lk = (lane >> 5) * 8;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int buf = (int)(kt & 1);
+    if (kt + 1 < nk) load_slice((kt + 1) * kGramBK);
+#pragma unroll
+    for (int kk = 0; kk < kGramBK; kk += 16) {
+      bf16x8 fa[2], fb[4];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        fa[p] = *reinterpret_cast<const bf16x8*>(&sA[buf][(wr + 32 * p + lrow) * kGramPitch + kk + lk]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        fb[q] = *reinterpret_cast<const bf16x8*>(&sB[buf][(wc + 32 * q + lrow) * kGramPitch + kk + lk]);
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[p][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[p], fb[q], acc[p][q], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store_slice(buf ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int32_t i = i0 + wr + 32 * p + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+        const int32_t j = j0 + wc + 32 * q + lrow;
+        if (i <= j && j < F) {
+          G[(int64_t)i * F + j] = acc[p][q][v];
+          if (i != j) G[(int64_t)j * F + i] = acc[p][q][v];
         }
       }
 }
@@ -191,16 +318,25 @@ hipError_t launch_matvec(const float* X, int64_t ld, int64_t row0, int64_t nrows
 }
 
 hipError_t launch_center_bf16(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
-                              int32_t F, int32_t Fp, uint16_t* XcT, int64_t Tp, hipStream_t s) {
+                              int32_t F, int32_t Fp, uint16_t* XcT, int64_t Tp, uint16_t* XcT_lo, hipStream_t s) {
   const dim3 grid(Fp / 64, (unsigned)(Tp / 64)), block(256);
-  center_bf16_kernel<<<grid, block, 0, s>>>(X, ld, row0, nrows, mean, F, XcT, Tp);
+  center_bf16_kernel<<<grid, block, 0, s>>>(X, ld, row0, nrows, mean, F, XcT, Tp, XcT_lo);
   return hipGetLastError();
 }
 
 hipError_t launch_gram(const uint16_t* XcT, int64_t Tp, int32_t Fp, int32_t F, float* G, hipStream_t s) {
+  // Fp is a multiple of 256 and Tp of 64 (both zero-padded by center_bf16_kernel)
+  const int32_t nI = Fp / kGramBM, nJ = Fp / kGramBN;
+  int32_t tiles = 0;
+  for (int32_t I = 0; I < nI; ++I) tiles += nJ - I / 2;
+  gram_kernel<<<dim3(tiles), dim3(256), 0, s>>>(XcT, Tp, nI, nJ, F, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_gram_split(const uint16_t* Hi, const uint16_t* Lo, int64_t Tp, int32_t Fp, int32_t F, float* G,
+                             hipStream_t s) {
   const int32_t nt = Fp / 128;
-  const dim3 grid(nt * (nt + 1) / 2), block(256);
-  gram_kernel<<<grid, block, 0, s>>>(XcT, Tp, nt, F, G);
+  gram_split_kernel<<<dim3(nt * (nt + 1) / 2), dim3(256), 0, s>>>(Hi, Lo, Tp, nt, F, G);
   return hipGetLastError();
 }
 

@@ -246,13 +246,15 @@ class LeviathanEngine:
             _lib.check("features_matvec")
         return out
 
-    def features_gram(self, row0: int, nrows: int, mean):
-        """(G, ms): BF16-input, FP32-accumulated Gram of the centred features, and the Gram kernel's time."""
+    def features_gram(self, row0: int, nrows: int, mean, terms: int = 1):
+        """(G, ms): Gram of the centred features on the matrix cores (BF16 inputs, FP32
+        accumulation; terms=3 uses split hi/lo BF16 inputs), and the Gram kernel's time."""
         F = 2 * self.n_feat_nodes
         mean = _arr(mean, np.float64, F, "mean")
         g = np.empty((F, F), dtype=np.float32)
         ms = ctypes.c_double(0.0)
-        if _lib.lib().leviathan_features_gram(self._h, int(row0), int(nrows), _p(mean), _p(g), ctypes.byref(ms)) != 0:
+        if _lib.lib().leviathan_features_gram(self._h, int(row0), int(nrows), _p(mean), int(terms), _p(g),
+                                              ctypes.byref(ms)) != 0:
             _lib.check("features_gram")
         return g, ms.value
 

@@ -83,6 +83,32 @@ def test_mfma_gram_matches_fp64_gram_of_bf16_inputs(res):
     assert np.abs(G - exact).max() / scale.max() < 2e-2
 
 
+def test_split_gram_matches_fp64_gram_of_hi_lo_parts(res):
+    """terms=3: the Gram of split BF16 inputs equals hi hi^T + hi lo^T + lo hi^T in FP64 up to
+    FP32 accumulation, and approximates the exact centred Gram ~2^8 times closer than one BF16
+    product (F = 300 padded to 384 columns, T = 2,900 rows padded to 2,944)."""
+    say("split gram")
+    r = res["res"]
+    row0, n = 300, 2900
+    X = r.features(row0, n).astype(np.float64)
+    mean = X.mean(axis=0)
+    G, ms = r.engine.features_gram(row0, n, mean, terms=3)
+    xc = (X - mean).astype(np.float32)
+    hi = oracle.bf16_round(xc)
+    lo = oracle.bf16_round(xc - hi)
+    H, L = hi.astype(np.float64), lo.astype(np.float64)
+    ref = H.T @ H + H.T @ L + L.T @ H
+    scale = np.sqrt(np.outer(np.diag(ref), np.diag(ref))).max()
+    assert np.abs(G - ref).max() / scale < 1e-5
+    assert np.array_equal(G, G.T)
+    exact = (X - mean).T @ (X - mean)
+    err3 = np.abs(G - exact).max() / scale
+    G1, _ = r.engine.features_gram(row0, n, mean, terms=1)
+    err1 = np.abs(G1 - exact).max() / scale
+    say("split gram error", err3, "bf16 gram error", err1)
+    assert err3 < 2e-5 and err3 < err1 / 20
+
+
 def test_ridge_fit_matches_fp64_oracle(res):
     """Preconditioned FP64 refinement reaches the FP64 ridge solution of the same features."""
     say("ridge fit")
@@ -90,8 +116,8 @@ def test_ridge_fit_matches_fp64_oracle(res):
     lag, row0 = 10, 200
     n = 3200 - row0 - lag
     y = u[row0 + lag: row0 + lag + n].astype(np.float64)
-    for alpha in (1e-4, 1.0):
-        fit = r.fit(y, row0, alpha=alpha, tol=1e-12)
+    for alpha, terms in ((1e-4, None), (1.0, 1), (1e-4, 3)):
+        fit = r.fit(y, row0, alpha=alpha, tol=1e-12, gram_terms=terms)
         X = r.features(row0, n)
         w_ref, c_ref = oracle.ridge_fp64(X, y, alpha)
         pred = r.predict(fit, row0, n)
