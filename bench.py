@@ -3,9 +3,12 @@
 
 Default workload: Barabasi-Albert graph N=10,000,000, m=10 (heavy-tailed
 in-degree, ~2.0e8 incoming edges), delays uniform in [1, 50], FP32 weights
-and phases.  The step is run with the uint16 exact-wrap history (the headline
-form: ``value``) and with the plain FP32 history (``plain_fp32``), each from
-the same seeded inputs already resident in HBM.
+and phases.  The step is run in three forms from the same seeded inputs
+already resident in HBM: the uint16 exact-wrap history with the autotuned
+schedule (the headline form: ``value``), the plain FP32 drop-in exactly as
+``leviathan_init`` configures it (FP32 history, direct schedule, 8 edges per
+lane: ``plain_fp32``), and the FP32 history with the autotuned schedule
+(``fp32_tuned``).
 
 One JSON line on stdout.  ``value`` = edge-steps/s (true nnz x steps / s) of
 the uint16 form; ``hbm_gbs`` = algorithmic bytes per step x steps/s.  The
@@ -160,12 +163,13 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
     ap.add_argument("--layout", default="node", choices=["node", "slot"])
     ap.add_argument("--ept", type=int, default=8)
-    ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned"])
+    ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned", "staged"])
     ap.add_argument("--bucket-edges", type=int, default=0)
     ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="f32,u16")
+    ap.add_argument("--forms", default="f32,f32-tuned,u16",
+                    help="f32: plain drop-in (direct/8); f32-tuned, u16: --schedule (default autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (profiler counter scaling)")
     args = ap.parse_args()
@@ -194,15 +198,20 @@ def main():
     wl = make_workload(args.workload, args.n)
     log(f"workload {wl.name}: N={wl.N} E={wl.E} built in {time.perf_counter() - t0:.1f} s")
     forms = {}
-    for h in args.forms.split(","):
-        forms[h] = run_form(wl, h, args.steps, args.warmup, args.layout, args.ept, args.schedule, args.bucket_edges)
-        log(h, json.dumps(forms[h]))
-    head = forms.get("u16") or forms["f32"]
+    for f in args.forms.split(","):
+        if f == "f32":
+            forms[f] = run_form(wl, "f32", args.steps, args.warmup, "node", 8, "direct")
+        else:
+            hist = "f32" if f == "f32-tuned" else f
+            forms[f] = run_form(wl, hist, args.steps, args.warmup, args.layout, args.ept, args.schedule,
+                                args.bucket_edges)
+        log(f, json.dumps(forms[f]))
+    head = forms.get("u16") or forms.get("f32-tuned") or forms["f32"]
     hist = head["history"]
     alg = algorithmic_bytes(wl.N, wl.E, hist)
-    # the step's kernels: the fused step kernel (direct) or scatter + bucket kernels (binned)
+    # the step's kernels: the fused step kernel (direct) or scatter + step kernels (two-phase schedules)
     step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"]
-    kernels = "scatter_kernel+bucket_kernel" if head["schedule"] == "binned" else "step_kernel"
+    kernels = "step_kernel" if head["schedule"].startswith("direct") else "scatter_kernel+step_kernel"
     achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -224,9 +233,12 @@ def main():
                    "history": hist, "layout": args.layout, "schedule": head["schedule"]},
         "hbm_gbs": head["hbm_gbs"],
         "plain_fp32": forms.get("f32"),
+        "fp32_tuned": forms.get("f32-tuned"),
         "u16": forms.get("u16"),
         "speedup_u16_vs_fp32": (forms["u16"]["edge_steps_per_s"] / forms["f32"]["edge_steps_per_s"]
                                 if "u16" in forms and "f32" in forms else None),
+        "speedup_u16_vs_fp32_tuned": (forms["u16"]["edge_steps_per_s"] / forms["f32-tuned"]["edge_steps_per_s"]
+                                      if "u16" in forms and "f32-tuned" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(args.workload, hist, head["schedule"]),

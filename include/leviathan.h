@@ -107,10 +107,15 @@ typedef struct {
 
 enum {
   LEV_SCHEDULE_DIRECT = 0, /* one fused kernel; delayed phases gathered per edge */
-  LEV_SCHEDULE_BINNED = 1, /* two phases: source-ordered scatter into destination buckets,
-                              then the fused row update from LDS (graphs without locality) */
-  LEV_SCHEDULE_AUTO = 2    /* both plans are built; leviathan_autotune picks */
+  LEV_SCHEDULE_BINNED = 1, /* two phases: source-ordered scatter into ~1 MB destination buckets,
+                              then the fused row update reading them (graphs without locality) */
+  LEV_SCHEDULE_AUTO = 2,   /* every plan is built; leviathan_autotune picks */
+  LEV_SCHEDULE_STAGED = 3  /* two phases: source-ordered scatter into each row block's own
+                              range, which the fused row update stages through LDS */
 };
+
+/* Plans a handle can hold: direct, binned, staged x 4, 8, 16 edges per lane. */
+#define LEV_NUM_PLANS 9
 
 /* Fills cfg with the defaults above (FP32 node-major history, 8 edges per
  * thread, 100-step graph chunks, direct schedule). */
@@ -164,13 +169,14 @@ int leviathan_num_blocks(LeviathanHandle* handle);
  * Returns 0 or -1. */
 int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* out);
 
-/* Schedule control (handles created with LEV_SCHEDULE_BINNED or _AUTO hold
- * every plan: direct and binned, each with 4, 8 and 16 edges per lane in the
- * row pass).  leviathan_set_schedule switches between them;
- * leviathan_autotune times `steps` steps of every built plan from the same
- * state, restores the state, keeps the fastest and writes the mean
- * milliseconds per step to ms_out[6] (direct/4, direct/8, direct/16,
- * binned/4, binned/8, binned/16; NaN for a plan not built).  All plans give
+/* Schedule control (a handle created with LEV_SCHEDULE_BINNED or _STAGED
+ * holds that schedule's plans and the direct ones, with _AUTO all of them;
+ * each has 4, 8 and 16 edges per lane in the row pass).
+ * leviathan_set_schedule switches between them; leviathan_autotune times
+ * `steps` steps of every built plan from the same state, restores the state,
+ * keeps the fastest and writes the mean milliseconds per step to
+ * ms_out[LEV_NUM_PLANS] (direct/4, direct/8, direct/16, binned/4, binned/8,
+ * binned/16, staged/4, staged/8, staged/16; NaN for a plan not built).  All plans give
  * the same phases and weights (the per-row arithmetic and summation order
  * are shared); r differs only in the grouping of its FP64 partial sums.
  * Return 0 or -1. */

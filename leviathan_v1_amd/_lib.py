@@ -41,7 +41,8 @@ class LeviathanConfig(ctypes.Structure):
 
 HISTORY = {"f32": 0, "u16": 1}
 LAYOUT = {"node": 0, "slot": 1}
-SCHEDULE = {"direct": 0, "binned": 1, "auto": 2}
+SCHEDULE = {"direct": 0, "binned": 1, "auto": 2, "staged": 3}
+PLAN_NAMES = [f"{k}/{e}" for k in ("direct", "binned", "staged") for e in (4, 8, 16)]
 
 # name -> (restype, argtypes); every symbol include/leviathan.h declares.
 SIGNATURES = {

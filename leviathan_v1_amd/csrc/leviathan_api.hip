@@ -52,8 +52,11 @@ struct GraphEntry {
 };
 
 // One schedule's work decomposition on the device.
-// Plans 0-2: direct with 4, 8, 16 edges per lane; 3-5: binned, same row blocks.
-constexpr int kPlanDirect4 = 0, kPlanDirect8 = 1, kPlanDirect16 = 2, kPlanBinned = 3, kNumPlans = 6;
+// Plans 0-2: direct with 4, 8, 16 edges per lane; 3-5: binned and 6-8:
+// staged, on the same row blocks.
+constexpr int kPlanDirect4 = 0, kPlanDirect8 = 1, kPlanDirect16 = 2, kPlanBinned = 3, kPlanStaged = 6;
+static_assert(LEV_NUM_PLANS == 9, "plan table");
+constexpr int kNumPlans = LEV_NUM_PLANS;
 
 struct DevPlan {
   bool built = false;
@@ -62,11 +65,15 @@ struct DevPlan {
   int4* d_items = nullptr;
   int32_t n_items = 0;
   int2* d_ranges = nullptr;
-  bool owns = true;  // binned plans share the row blocks of the direct plan with the same ept
+  bool owns = true;            // binned and staged plans share the row blocks of the direct plan with the same ept
+  uint32_t* d_pos = nullptr;   // two-phase plans: [E] value slots in the scatter kernel's write order
+  uint16_t* d_rank = nullptr;  // two-phase plans: [E] CSC edge -> index in its source block's write order
+  uint16_t* d_lpos = nullptr;  // staged plans: [E] CSR edge -> slot within its block's range
+  bool owns_pos = false;
 };
 
 int plan_of_ept(int ept) { return ept == 4 ? kPlanDirect4 : ept == 16 ? kPlanDirect16 : kPlanDirect8; }
-const char* const kPlanNames[kNumPlans] = {"direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"};
+int plan_base(int kind) { return kind == LEV_SCHEDULE_BINNED ? kPlanBinned : kind == LEV_SCHEDULE_STAGED ? kPlanStaged : 0; }
 
 }  // namespace
 
@@ -101,6 +108,7 @@ struct LeviathanHandle {
   uint32_t* d_pos = nullptr;
   void* d_hbuf = nullptr;
   uint32_t* d_hpos = nullptr;
+  uint16_t* d_rank = nullptr;
   int4* d_sources = nullptr;  // phase-1 work items
   int32_t n_sources = 0;
   int32_t n_buckets = 0;
@@ -156,11 +164,16 @@ void destroy(LeviathanHandle* h) {
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
-                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_sources,  h->d_feat_nodes,
+                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
+    if (pl.owns_pos) {
+      void* pp[] = {pl.d_pos, pl.d_rank, pl.d_lpos};
+      for (void* p : pp)
+        if (p) (void)hipFree(p);
+    }
     if (!pl.owns) continue;
     void* pp[] = {pl.d_items, pl.d_ranges};
     for (void* p : pp)
@@ -207,6 +220,7 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.drive_u = h->d_drive_u;
   a.hbuf = h->d_hbuf;
   a.hpos = h->d_hpos;
+  a.lpos = pl.d_lpos;
   a.dt = dt;
   a.eta = h->cfg.eta;
   a.lam = h->cfg.lam;
@@ -235,11 +249,11 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   lev::ScatterArgs sa{};
   sa.csc_ptr = h->d_csc_ptr;
   sa.delay = h->d_delay_csc;
-  sa.pos = h->d_pos;
+  sa.pos = pl.d_pos;
+  sa.rank = pl.d_rank;
   sa.ring = h->d_ring;
   sa.ring_pitch = h->ring_pitch;
   sa.n_slots = h->n_slots;
-  (void)pl;
   sa.num_blocks = h->n_sources;
   sa.xcd_remap = 1;
   sa.blocks = h->d_sources;
@@ -255,7 +269,9 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr) 
     return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
   hipError_t e = lev::launch_scatter(h->cfg.history, h->cfg.layout, scatter_args(h, pl), h->stream);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
-  if (e == hipSuccess) e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 1, step_args(h, dt, pl), h->stream);
+  if (e == hipSuccess)
+    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1,
+                         step_args(h, dt, pl), h->stream);
   return e;
 }
 
@@ -419,7 +435,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     set_err("graph_chunk must be in [0, %d]", lev::kRHistSteps);
     return nullptr;
   }
-  if (cfg.schedule != LEV_SCHEDULE_DIRECT && cfg.schedule != LEV_SCHEDULE_BINNED && cfg.schedule != LEV_SCHEDULE_AUTO) {
+  if (cfg.schedule < LEV_SCHEDULE_DIRECT || cfg.schedule > LEV_SCHEDULE_STAGED) {
     set_err("unknown schedule %d", cfg.schedule);
     return nullptr;
   }
@@ -513,12 +529,26 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     h->plans[q].ept = epts[q];
     h->plans[q].n_items = (int32_t)hp[q].items.size();
   }
-  lev::HostBinned hb;
-  if (cfg.schedule != LEV_SCHEDULE_DIRECT) {
-    if (!lev::build_binned_plan(N, row_ptr, col_idx, delays, n_nets, offs.data(), cfg.bucket_edges, hb, perr)) {
+  // two-phase plans: the scatter plan over sources, and per plan the value slots
+  lev::HostScatter hsc;
+  std::vector<uint32_t> bin_pos, bin_hpos, st_pos[3];
+  std::vector<uint16_t> bin_rank, st_rank[3], st_lpos[3];
+  const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
+  const bool want_staged = cfg.schedule == LEV_SCHEDULE_STAGED || cfg.schedule == LEV_SCHEDULE_AUTO;
+  if (want_binned || want_staged) {
+    if (!lev::build_scatter_plan(N, row_ptr, col_idx, delays, hsc, perr)) {
       set_err("%s", perr.c_str());
       return fail();
     }
+    h->n_sources = (int32_t)hsc.sources.size();
+  }
+  if (want_binned) {
+    if (!lev::bucket_positions(N, row_ptr, col_idx, cfg.bucket_edges, bin_hpos, h->n_buckets, perr) ||
+        !lev::csc_positions(hsc, bin_hpos, bin_pos, perr)) {
+      set_err("%s", perr.c_str());
+      return fail();
+    }
+    lev::sort_scatter_writes(hsc, bin_pos, bin_rank);
     for (int q = 0; q < 3; ++q) {
       DevPlan& pb = h->plans[kPlanBinned + q];
       pb.built = true;
@@ -527,10 +557,27 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       pb.n_items = h->plans[q].n_items;
       pb.owns = false;
     }
-    h->n_sources = (int32_t)hb.sources.size();
-    h->n_buckets = hb.n_buckets;
   }
-  h->active = (cfg.schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : 0) + plan_of_ept(cfg.edges_per_thread);
+  if (want_staged) {
+    for (int q = 0; q < 3; ++q) {
+      std::vector<uint32_t> hpos;
+      if (!lev::staged_positions(N, row_ptr, col_idx, hp[q], hpos, st_lpos[q], perr) ||
+          !lev::csc_positions(hsc, hpos, st_pos[q], perr)) {
+        set_err("%s", perr.c_str());
+        return fail();
+      }
+      lev::sort_scatter_writes(hsc, st_pos[q], st_rank[q]);
+      DevPlan& ps = h->plans[kPlanStaged + q];
+      ps.built = true;
+      ps.kind = LEV_SCHEDULE_STAGED;
+      ps.ept = epts[q];
+      ps.n_items = h->plans[q].n_items;
+      ps.owns = false;
+    }
+  }
+  hsc.csc_edge = std::vector<uint32_t>();
+  h->active = plan_base(cfg.schedule == LEV_SCHEDULE_AUTO ? LEV_SCHEDULE_DIRECT : cfg.schedule) +
+              plan_of_ept(cfg.edges_per_thread);
   for (const auto& pl : h->plans)
     if (pl.built) h->n_partials = std::max(h->n_partials, pl.n_items);
   for (int k = 0; k < n_nets; ++k) {
@@ -586,22 +633,43 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         !hip_ok(hipMemcpy(pl.d_ranges, hp[q].ranges.data(), sizeof(int2) * n_nets, H2D), "copy ranges"))
       return fail();
   }
-  if (h->plans[kPlanBinned].built) {
-    for (int q = 0; q < 3; ++q) {
-      h->plans[kPlanBinned + q].d_items = h->plans[q].d_items;
-      h->plans[kPlanBinned + q].d_ranges = h->plans[q].d_ranges;
-    }
+  if (want_binned || want_staged) {
     const size_t E = (size_t)h->E;
-    if (!dalloc(h, &h->d_sources, hb.sources.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
-        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) ||
-        !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes))
+    if (!dalloc(h, &h->d_sources, hsc.sources.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
+        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes))
       return fail();
-    ok = hip_ok(hipMemcpy(h->d_sources, hb.sources.data(), sizeof(int4) * hb.sources.size(), H2D), "copy sources") &&
-         hip_ok(hipMemcpy(h->d_csc_ptr, hb.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
-         (E == 0 || (hip_ok(hipMemcpy(h->d_delay_csc, hb.delay_csc.data(), E, H2D), "copy csc delays") &&
-                     hip_ok(hipMemcpy(h->d_pos, hb.pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
-                     hip_ok(hipMemcpy(h->d_hpos, hb.hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots")));
+    ok = hip_ok(hipMemcpy(h->d_sources, hsc.sources.data(), sizeof(int4) * hsc.sources.size(), H2D),
+                "copy sources") &&
+         hip_ok(hipMemcpy(h->d_csc_ptr, hsc.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
+         (E == 0 || hip_ok(hipMemcpy(h->d_delay_csc, hsc.delay_csc.data(), E, H2D), "copy csc delays"));
     if (!ok) return fail();
+    if (want_binned) {
+      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E)) return fail();
+      if (E && !(hip_ok(hipMemcpy(h->d_pos, bin_pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
+                 hip_ok(hipMemcpy(h->d_hpos, bin_hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots") &&
+                 hip_ok(hipMemcpy(h->d_rank, bin_rank.data(), sizeof(uint16_t) * E, H2D), "copy ranks")))
+        return fail();
+      for (int q = 0; q < 3; ++q) {
+        DevPlan& pb = h->plans[kPlanBinned + q];
+        pb.d_items = h->plans[q].d_items;
+        pb.d_ranges = h->plans[q].d_ranges;
+        pb.d_pos = h->d_pos;
+        pb.d_rank = h->d_rank;
+      }
+    }
+    if (want_staged) {
+      for (int q = 0; q < 3; ++q) {
+        DevPlan& ps = h->plans[kPlanStaged + q];
+        ps.d_items = h->plans[q].d_items;
+        ps.d_ranges = h->plans[q].d_ranges;
+        ps.owns_pos = true;
+        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E)) return fail();
+        if (E && !(hip_ok(hipMemcpy(ps.d_pos, st_pos[q].data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
+                   hip_ok(hipMemcpy(ps.d_rank, st_rank[q].data(), sizeof(uint16_t) * E, H2D), "copy ranks") &&
+                   hip_ok(hipMemcpy(ps.d_lpos, st_lpos[q].data(), sizeof(uint16_t) * E, H2D), "copy slots")))
+          return fail();
+      }
+    }
   }
   // history pre-fill (init_buffer_kernel, leviathan_kernel.cu:371-372)
   if (!hip_ok(lev::launch_fill_ring(cfg.history, cfg.layout, h->d_ring, h->ring_pitch, h->n_slots, h->d_theta, N,
@@ -897,7 +965,7 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
     set_err("profile_steps runs undriven steps; switch the drive off first");
     return -1;
   }
-  const bool binned = h->plans[h->active].kind == LEV_SCHEDULE_BINNED;
+  const bool binned = h->plans[h->active].kind != LEV_SCHEDULE_DIRECT;
   // per step: [start, (mid: after scatter), after main, after finalize]
   std::vector<hipEvent_t> ev((size_t)4 * n, nullptr);
   bool ok = true;
@@ -943,11 +1011,12 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
 
 int leviathan_set_schedule(LeviathanHandle* h, int schedule, int ept) {
   if (!check_handle(h)) return -1;
-  if ((schedule != LEV_SCHEDULE_DIRECT && schedule != LEV_SCHEDULE_BINNED) || (ept != 4 && ept != 8 && ept != 16)) {
-    set_err("schedule must be DIRECT or BINNED with edges_per_thread 4, 8 or 16");
+  if ((schedule != LEV_SCHEDULE_DIRECT && schedule != LEV_SCHEDULE_BINNED && schedule != LEV_SCHEDULE_STAGED) ||
+      (ept != 4 && ept != 8 && ept != 16)) {
+    set_err("schedule must be DIRECT, BINNED or STAGED with edges_per_thread 4, 8 or 16");
     return -1;
   }
-  const int idx = (schedule == LEV_SCHEDULE_BINNED ? kPlanBinned : 0) + plan_of_ept(ept);
+  const int idx = plan_base(schedule) + plan_of_ept(ept);
   if (!h->plans[idx].built) {
     set_err("that schedule was not built for this handle (create it with LEV_SCHEDULE_AUTO)");
     return -1;

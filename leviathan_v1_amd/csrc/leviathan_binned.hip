@@ -10,7 +10,10 @@
 //   phase 1, scatter_kernel (this file): walk the graph by SOURCE (CSC order).
 //     Lanes on consecutive out-edges of a run of consecutive sources read the
 //     sources' history once and write h_e = history[j][t - d_e] into a
-//     destination-bucketed array.  A bucket holds the incoming edges of a run
+//     destination-bucketed array.  The block first places its values in LDS
+//     in ascending slot order (a host-computed rank per edge): its values for
+//     one bucket are one contiguous run of slots, so they leave as a few
+//     whole-run writes rather than one L2 request per edge.  A bucket holds the incoming edges of a run
 //     of consecutive rows (about 1 MB of values), ordered by source, so each
 //     bucket has one write front per XCD that advances with the source sweep;
 //     with a few hundred buckets the fronts stay in L2 until their lines are
@@ -21,7 +24,7 @@
 //     contiguous run of row blocks, so the bucket being read (1 MB) stays in
 //     that XCD's 4 MB L2 and is fetched from HBM about once.
 //
-// Per edge the HBM traffic is delay 1 + position 4 + h write 2|4 (phase 1) and
+// Per edge the HBM traffic is delay 1 + rank 2 + position 4 + h write 2|4 (phase 1) and
 // position 4 + h read 2|4 + weight read/write 8 (phase 2), plus one pass over
 // the history ring, instead of a cache line per edge.
 #include <hip/hip_runtime.h>
@@ -37,6 +40,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   constexpr int EPT = kScatterEpt;
   constexpr int CAP = kBlock * EPT;
   __shared__ uint8_t s_esrc[CAP];
+  __shared__ H s_val[CAP];  // the block's values in slot order
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
   const int4 blk = a.blocks[b];
@@ -60,15 +64,18 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     }
     __syncthreads();
     int32_t dd[EPT];
+    int32_t rk[EPT];
     uint32_t pp[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) {
         dd[k] = __builtin_nontemporal_load(a.delay + c0 + le);
+        rk[k] = __builtin_nontemporal_load(a.rank + c0 + le);
         pp[k] = __builtin_nontemporal_load(a.pos + c0 + le);
       }
     }
+    // read by source (lanes of one source share its history line), place in slot order
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
@@ -78,8 +85,15 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         // node-major: a source's history line is only reused by this block's lanes, so it is
         // read with the streaming hint and leaves the L2 to the buckets' write fronts
         const H* src = ring + ring_index<LAYOUT>(pitch, s0 + s_esrc[le], slot);
-        hbuf[pp[k]] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+        s_val[rk[k]] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
       }
+    }
+    __syncthreads();
+    // write in slot order: the values bound for one bucket leave as one contiguous run
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      if (le < nnz) hbuf[pp[k]] = s_val[le];
     }
   } else {
     // one source with more than CAP out-edges: the whole workgroup strides over them

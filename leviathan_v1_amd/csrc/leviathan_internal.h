@@ -15,6 +15,7 @@ constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-r
 
 // Row block flag: a single row whose in-degree exceeds the block capacity.
 constexpr int kBlockLongRow = 1;
+constexpr int kSerialRow = 32;       // stream blocks: longer rows are filled and summed by a wavefront
 
 // Per-network controller state, read by the step kernel (g) and updated by
 // the finalize kernel (r, g) in FP64.
@@ -54,6 +55,7 @@ struct StepArgs {
   const float* drive_u;   // [kDriveSteps] ring, u_t at t % kDriveSteps
   const void* hbuf;       // binned schedule: delayed phases written by the scatter kernel
   const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
+  const uint16_t* lpos;   // staged schedule: [E] position of each CSR edge's value in its block's range
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
 
@@ -74,7 +76,8 @@ constexpr int kScatterEpt = 8;      // phase 1: out-edges per lane in a source b
 struct ScatterArgs {
   const int32_t* csc_ptr;   // [N+1] out-edge offsets per source
   const uint8_t* delay;     // [E] delays in CSC order
-  const uint32_t* pos;      // [E] position of each out-edge in the bucketed h array
+  const uint32_t* pos;      // [E] value slots: per short source block in slot order, per long source in CSC order
+  const uint16_t* rank;     // [E] short source blocks: CSC edge -> its index in the block's slot order
   const void* ring;
   int64_t ring_pitch;
   int32_t n_slots;

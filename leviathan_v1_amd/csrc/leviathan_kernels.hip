@@ -35,6 +35,11 @@ namespace lev {
 // GATHER = 0: the delayed phase of edge e is read from the history ring at
 // (col[e], t - delay[e]).  GATHER = 1: it was written by the scatter kernel
 // to hbuf[hpos[e]] (binned schedule); col and delay are not read.
+// GATHER = 2 (staged schedule): the scatter kernel wrote the values of a row
+// block's edges into the block's own range [e0, e1) of hbuf, ordered by
+// source.  The block reads that range with coalesced loads into LDS and picks
+// edge e's value at e0 + lpos[e] there; a long row's range is in CSR order and
+// is read straight.
 template <class H, int LAYOUT, int EPT, int GATHER>
 __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   constexpr int CAP = kBlock * EPT;
@@ -77,9 +82,19 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       s_gate[tid] = rs_.gate;
       rs = a.row_ptr[i] - e0;
       re = a.row_ptr[i + 1] - e0;
-      for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
     }
-    __syncthreads();
+    // Rows of up to kSerialRow edges are handled by their own lane, longer
+    // ones by their whole wavefront, one after another (a lane looping over a
+    // long row would hold its wavefront for the row's whole length).
+    const int lane = tid & 63;
+    const bool heavy = tid < nrows && re - rs > kSerialRow;
+    if (tid < nrows && !heavy)
+      for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
+    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      const int32_t hs = __shfl(rs, src, 64), he = __shfl(re, src, 64);
+      for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
+    }
 
     int32_t jj[EPT];
     int32_t dd[EPT];
@@ -91,12 +106,24 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         if constexpr (GATHER == 0) {
           jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
           dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
-        } else {
+        } else if constexpr (GATHER == 1) {
           jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+        } else {
+          jj[k] = (int32_t)__builtin_nontemporal_load(a.lpos + e0 + le);
         }
         ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
       }
     }
+    if constexpr (GATHER == 2) {
+      // s_contrib holds the block's staged values until the edge pass below
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(__builtin_nontemporal_load(hbuf + e0 + le));
+      }
+    }
+    __syncthreads();
+
     float hh[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
@@ -106,11 +133,14 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           int32_t slot = cur - dd[k];
           slot += (slot < 0) ? S : 0;
           hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
-        } else {
+        } else if constexpr (GATHER == 1) {
           hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
+        } else {
+          hh[k] = s_contrib[jj[k]];
         }
       }
     }
+    if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
@@ -124,19 +154,34 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     }
     __syncthreads();
 
-    if (tid < nrows) {
-      const int32_t i = r0 + tid;
+    // Row sums in CSR order: Neumaier's sum by the row's lane, or for a long
+    // row lane-strided Neumaier partials combined by a fixed FP64 tree.
+    float coupling = 0.0f;
+    if (tid < nrows && !heavy) {
       CompSum acc;
       for (int32_t q = rs; q < re; ++q) acc.add(s_contrib[q]);
+      coupling = acc.value();
+    }
+    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      const int32_t hs = __shfl(rs, src, 64), he = __shfl(re, src, 64);
+      CompSum acc;
+      for (int32_t q = hs + lane; q < he; q += 64) acc.add(s_contrib[q]);
+      const double tot = wave_sum((double)acc.s + (double)acc.c);
+      if (lane == src) coupling = (float)tot;
+    }
+
+    if (tid < nrows) {
+      const int32_t i = r0 + tid;
       float om = a.omega[i];
       if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
-      const float x = row_integrate(rs_, om, g, acc.value(), dt, a.alpha);
+      const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
       a.theta_hat[i] = rs_.thh_new;
       a.omega_hat[i] = rs_.omh_new;
       float sx, cx;
-      sincosf(x, &sx, &cx);
+      sincos_fast(x, &sx, &cx);
       my_c = (double)cx;
       my_s = (double)sx;
     }
@@ -163,8 +208,10 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           if constexpr (GATHER == 0) {
             jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
             dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
-          } else {
+          } else if constexpr (GATHER == 1) {
             jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+          } else {
+            jj[k] = 0;
           }
           ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
         }
@@ -177,8 +224,10 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
             int32_t slot = cur - dd[k];
             slot += (slot < 0) ? S : 0;
             hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
-          } else {
+          } else if constexpr (GATHER == 1) {
             hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
+          } else {
+            hh[k] = HistCodec<H>::decode(__builtin_nontemporal_load(hbuf + e0 + le));
           }
         }
       }
@@ -209,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       a.theta_hat[i] = rs_.thh_new;
       a.omega_hat[i] = rs_.omh_new;
       float sx, cx;
-      sincosf(x, &sx, &cx);
+      sincos_fast(x, &sx, &cx);
       my_c = (double)cx;
       my_s = (double)sx;
     }
@@ -346,7 +395,16 @@ static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
 
 template <class H, int LAYOUT>
 static hipError_t launch_step_g(int ept, int gather, const StepArgs& a, hipStream_t s) {
-  return gather ? launch_step_t<H, LAYOUT, 1>(ept, a, s) : launch_step_t<H, LAYOUT, 0>(ept, a, s);
+  switch (gather) {
+    case 0:
+      return launch_step_t<H, LAYOUT, 0>(ept, a, s);
+    case 1:
+      return launch_step_t<H, LAYOUT, 1>(ept, a, s);
+    case 2:
+      return launch_step_t<H, LAYOUT, 2>(ept, a, s);
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {

@@ -2,6 +2,7 @@
 #include "plan.h"
 
 #include <algorithm>
+#include <atomic>
 #include <thread>
 
 #include "leviathan_internal.h"
@@ -67,67 +68,26 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
   return true;
 }
 
-bool build_binned_plan(int N, const int* row_ptr, const int* col, const uint8_t* delays, int n_nets,
-                       const int* offs, int bucket_edges, HostBinned& out, std::string& err) {
+bool build_scatter_plan(int N, const int* row_ptr, const int* col, const uint8_t* delays, HostScatter& out,
+                        std::string& err) {
   const int64_t E = row_ptr[N];
-  if (bucket_edges < 1024) {
-    err = "bucket_edges must be at least 1024";
+  if (E > (int64_t)UINT32_MAX) {
+    err = "too many edges for 32-bit positions";
     return false;
   }
-  (void)n_nets;
-  (void)offs;
-  // 1. buckets over rows
-  std::vector<int32_t> bstart{0};
-  {
-    int r = 0;
-    while (r < N) {
-      int64_t nnz = deg_of(row_ptr, r);
-      ++r;
-      while (r < N && nnz + deg_of(row_ptr, r) <= bucket_edges) {
-        nnz += deg_of(row_ptr, r);
-        ++r;
-      }
-      bstart.push_back(r);
-    }
-  }
-  out.n_buckets = (int32_t)bstart.size() - 1;
-
-  // 2. slot of every CSR edge: bucket start + rank by (source, CSR index)
-  out.hpos.assign((size_t)E, 0);
-  parallel_for(out.n_buckets, [&](int64_t q) {
-    const int32_t e0 = row_ptr[bstart[q]], e1 = row_ptr[bstart[q + 1]];
-    const int32_t n = e1 - e0;
-    std::vector<uint64_t> key((size_t)n);
-    for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)(uint32_t)col[e0 + l] << 32) | (uint32_t)l;
-    std::sort(key.begin(), key.end());
-    for (int32_t k = 0; k < n; ++k) out.hpos[e0 + (uint32_t)(key[k] & 0xffffffffu)] = (uint32_t)(e0 + k);
-  });
-
-  // 3. CSC (by source), destinations ascending within a source
+  // CSC (by source), destinations ascending within a source
   out.csc_ptr.assign((size_t)N + 1, 0);
   for (int64_t e = 0; e < E; ++e) out.csc_ptr[col[e] + 1]++;
   for (int j = 0; j < N; ++j) out.csc_ptr[j + 1] += out.csc_ptr[j];
   std::vector<int32_t> cur(out.csc_ptr.begin(), out.csc_ptr.end() - 1);
-  out.pos.assign((size_t)E, 0);
+  out.csc_edge.assign((size_t)E, 0);
   out.delay_csc.assign((size_t)E, 0);
   for (int64_t e = 0; e < E; ++e) {
     const int32_t c = cur[col[e]]++;
-    out.pos[c] = out.hpos[e];
+    out.csc_edge[c] = (uint32_t)e;
     out.delay_csc[c] = delays[e];
   }
-  // every slot is written exactly once per step: pos is a permutation of [0, E)
-  {
-    std::vector<uint8_t> seen((size_t)E, 0);
-    for (int64_t c = 0; c < E; ++c) {
-      if (out.pos[c] >= (uint64_t)E || seen[out.pos[c]]) {
-        err = "internal error: bucket slots are not a permutation";
-        return false;
-      }
-      seen[out.pos[c]] = 1;
-    }
-  }
-
-  // 4. phase-1 work items over sources
+  // phase-1 work items over sources
   const int cap_src = kBlock * kScatterEpt;
   out.sources.clear();
   std::vector<int4> longs, shorts;
@@ -151,6 +111,107 @@ bool build_binned_plan(int N, const int* row_ptr, const int* col, const uint8_t*
   }
   out.sources.insert(out.sources.end(), longs.begin(), longs.end());
   out.sources.insert(out.sources.end(), shorts.begin(), shorts.end());
+  return true;
+}
+
+namespace {
+
+// hpos[e0 + l] for l in [0, n): e0 + the rank of edge e0 + l by (source, CSR index)
+void rank_by_source(const int* col, int32_t e0, int32_t n, uint32_t* hpos) {
+  std::vector<uint64_t> key((size_t)n);
+  for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)(uint32_t)col[e0 + l] << 32) | (uint32_t)l;
+  std::sort(key.begin(), key.end());
+  for (int32_t k = 0; k < n; ++k) hpos[e0 + (uint32_t)(key[k] & 0xffffffffu)] = (uint32_t)(e0 + k);
+}
+
+}  // namespace
+
+bool bucket_positions(int N, const int* row_ptr, const int* col, int bucket_edges, std::vector<uint32_t>& hpos,
+                      int32_t& n_buckets, std::string& err) {
+  const int64_t E = row_ptr[N];
+  if (bucket_edges < 1024) {
+    err = "bucket_edges must be at least 1024";
+    return false;
+  }
+  std::vector<int32_t> bstart{0};
+  int r = 0;
+  while (r < N) {
+    int64_t nnz = deg_of(row_ptr, r);
+    ++r;
+    while (r < N && nnz + deg_of(row_ptr, r) <= bucket_edges) {
+      nnz += deg_of(row_ptr, r);
+      ++r;
+    }
+    bstart.push_back(r);
+  }
+  n_buckets = (int32_t)bstart.size() - 1;
+  hpos.assign((size_t)E, 0);
+  parallel_for(n_buckets, [&](int64_t q) {
+    const int32_t e0 = row_ptr[bstart[q]];
+    rank_by_source(col, e0, row_ptr[bstart[q + 1]] - e0, hpos.data());
+  });
+  return true;
+}
+
+bool staged_positions(int N, const int* row_ptr, const int* col, const HostPlan& plan, std::vector<uint32_t>& hpos,
+                      std::vector<uint16_t>& lpos, std::string& err) {
+  const int64_t E = row_ptr[N];
+  hpos.assign((size_t)E, 0);
+  lpos.assign((size_t)E, 0);
+  std::atomic<bool> ok{true};
+  parallel_for((int64_t)plan.items.size(), [&](int64_t q) {
+    const int4 it = plan.items[q];
+    const int32_t e0 = row_ptr[it.x], e1 = row_ptr[it.y];
+    if (it.w & kBlockLongRow) {
+      for (int32_t e = e0; e < e1; ++e) hpos[e] = (uint32_t)e;  // read straight, in CSR order
+      return;
+    }
+    if (e1 - e0 > 65536) {
+      ok.store(false);
+      return;
+    }
+    rank_by_source(col, e0, e1 - e0, hpos.data());
+    for (int32_t e = e0; e < e1; ++e) lpos[e] = (uint16_t)(hpos[e] - (uint32_t)e0);
+  });
+  if (!ok.load()) err = "internal error: a row block holds more than 65536 edges";
+  return ok.load();
+}
+
+static_assert(kBlock * kScatterEpt <= 65536, "scatter ranks are 16-bit");
+
+void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std::vector<uint16_t>& rank) {
+  rank.assign(pos.size(), 0);
+  parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {
+    const int4 it = sc.sources[q];
+    if (it.w & kBlockLongRow) return;
+    const int32_t c0 = sc.csc_ptr[it.x], c1 = sc.csc_ptr[it.y];
+    const int32_t n = c1 - c0;
+    std::vector<uint64_t> key((size_t)n);
+    for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)pos[c0 + l] << 32) | (uint32_t)l;
+    std::sort(key.begin(), key.end());
+    for (int32_t k = 0; k < n; ++k) {
+      const uint32_t l = (uint32_t)(key[k] & 0xffffffffu);
+      rank[c0 + l] = (uint16_t)k;
+      pos[c0 + k] = (uint32_t)(key[k] >> 32);
+    }
+  });
+}
+
+bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std::vector<uint32_t>& pos,
+                   std::string& err) {
+  const size_t E = sc.csc_edge.size();
+  pos.assign(E, 0);
+  std::vector<uint8_t> seen(E, 0);
+  for (size_t c = 0; c < E; ++c) {
+    const uint32_t p = hpos[sc.csc_edge[c]];
+    // every slot is written exactly once per step: the positions are a permutation of [0, E)
+    if (p >= E || seen[p]) {
+      err = "internal error: value slots are not a permutation";
+      return false;
+    }
+    seen[p] = 1;
+    pos[c] = p;
+  }
   return true;
 }
 

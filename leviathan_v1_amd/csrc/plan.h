@@ -16,14 +16,12 @@ struct HostPlan {
   std::vector<int2> ranges;
 };
 
-// The two-phase (binned) schedule.
-struct HostBinned {
-  std::vector<int4> sources;        // phase-1 work items {source_begin, source_end, 0, flags}
+// Phase 1 of the two-phase schedules: the graph by source.
+struct HostScatter {
+  std::vector<int4> sources;        // work items {source_begin, source_end, 0, flags}
   std::vector<int32_t> csc_ptr;     // [N+1] out-edge offsets
   std::vector<uint8_t> delay_csc;   // [E] delays in CSC order
-  std::vector<uint32_t> pos;        // [E] CSC edge -> slot in the bucketed array
-  std::vector<uint32_t> hpos;       // [E] CSR edge -> slot in the bucketed array
-  int32_t n_buckets = 0;
+  std::vector<uint32_t> csc_edge;   // [E] CSC index -> CSR edge
 };
 
 // Row blocks of the direct schedule: runs of <= 256 rows holding <= cap edges,
@@ -32,10 +30,34 @@ struct HostBinned {
 bool build_direct_plan(const int* row_ptr, int n_nets, const int* net_offsets, int cap, HostPlan& out,
                        std::string& err);
 
-// Buckets: runs of consecutive rows with <= bucket_edges incoming edges (a
-// heavier row is a bucket by itself); inside a bucket the edges are ordered
-// by (source, CSR index).  Also builds the source-ordered (CSC) scatter plan.
-bool build_binned_plan(int N, const int* row_ptr, const int* col_idx, const uint8_t* delays, int n_nets,
-                       const int* net_offsets, int bucket_edges, HostBinned& out, std::string& err);
+// CSC of the graph and the phase-1 work items: runs of <= 256 sources with
+// <= kBlock * kScatterEpt out-edges, or one heavier source.
+bool build_scatter_plan(int N, const int* row_ptr, const int* col_idx, const uint8_t* delays, HostScatter& out,
+                        std::string& err);
+
+// Binned schedule: buckets are runs of consecutive rows with <= bucket_edges
+// incoming edges (a heavier row is a bucket by itself); inside a bucket the
+// values are ordered by (source, CSR index).  hpos[e] = slot of CSR edge e.
+bool bucket_positions(int N, const int* row_ptr, const int* col_idx, int bucket_edges, std::vector<uint32_t>& hpos,
+                      int32_t& n_buckets, std::string& err);
+
+// Staged schedule: each row block of `plan` owns its CSR edge range; a
+// stream block's values are ordered by (source, CSR index) inside it, a long
+// row's keep CSR order.  hpos[e] = slot of CSR edge e, lpos[e] = hpos[e] - the
+// block's first edge (stream blocks only).
+bool staged_positions(int N, const int* row_ptr, const int* col_idx, const HostPlan& plan,
+                      std::vector<uint32_t>& hpos, std::vector<uint16_t>& lpos, std::string& err);
+
+// pos[c] = hpos[CSR edge of CSC index c], checked to be a permutation of [0, E).
+bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std::vector<uint32_t>& pos,
+                   std::string& err);
+
+// Write order of the scatter kernel: inside each short source block the
+// values are written in ascending slot order, so the block's values for one
+// bucket or region (a contiguous run of slots, as slots are ordered by
+// source there) leave as one contiguous run.  On return pos holds, per short
+// block, the slots in ascending order, and rank[c] the index of CSC edge c in
+// that order; long sources keep CSC order (rank unused).
+void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std::vector<uint16_t>& rank);
 
 }  // namespace lev

@@ -6,6 +6,9 @@
 
 #include "leviathan_internal.h"
 
+#define LEV_HD __host__ __device__ __forceinline__
+#include "sincos_bounded.h"
+
 namespace lev {
 
 // 2*pi rounded to FP32 (the reference's 2.0f * CUDART_PI_F, leviathan_kernel.cu:102).
@@ -85,6 +88,14 @@ struct CompSum {
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
+// sin and cos of any float: the bounded routine, and the general one for
+// |x| beyond its range and NaN (a branch no wave takes while every phase is
+// wrapped).
+__device__ __forceinline__ void sincos_fast(float x, float* s, float* c) {
+  sincos_bounded(x, s, c);
+  if (!(fabsf(x) <= kSinCosBoundedMax)) sincosf(x, s, c);
+}
+
 // Per-row part of the step, shared by every kernel variant so the arithmetic
 // is written once.  The library is compiled with -ffp-contract=off and every
 // fused multiply-add is explicit, so all kernel variants round identically.
@@ -97,7 +108,8 @@ struct RowState {
 __device__ __forceinline__ RowState row_prep(float th, float thh, float omh, float dt, float kappa_w, float gamma) {
   RowState s;
   s.th = th;
-  s.eps = sinf(wrap_pm_pi_div(th - thh));
+  float unused;
+  sincos_fast(wrap_pm_pi_div(th - thh), &s.eps, &unused);
   s.thh_new = fmaf(dt, omh, thh);
   s.omh_new = fmaf(kappa_w * dt, s.eps, omh);
   s.gate = fmaf(-gamma, fabsf(s.eps), 1.0f);
@@ -119,7 +131,7 @@ __device__ __forceinline__ float edge_update(float h, float th_i, float gate, fl
                                              float w_min, float w_max, float* contrib) {
   const float diff = wrap_pm_pi(h - th_i);
   float sn, cs;
-  sincosf(diff, &sn, &cs);
+  sincos_fast(diff, &sn, &cs);
   *contrib = w * sn;
   float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
   if (!__builtin_isfinite(wv)) wv = w_min;

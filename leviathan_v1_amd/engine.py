@@ -50,7 +50,9 @@ class LeviathanEngine:
       edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
       schedule: "direct" (one fused kernel, per-edge gather), "binned" (source-
                ordered scatter into destination buckets, then the fused row
-               update) or "auto" (both built; autotune() picks)
+               update), "staged" (source-ordered scatter into each row block's
+               own range, staged through LDS by the row update) or "auto"
+               (all built; autotune() picks)
       bucket_edges: binned schedule bucket capacity (0: automatic)
       graph_chunk: steps per captured HIP graph in step_n (0 disables graphs)
       params: dict of physics parameters (names of LeviathanConfig)
@@ -306,11 +308,10 @@ class LeviathanEngine:
     def autotune(self, dt: float, steps: int = 10) -> dict:
         """Times every built schedule from the current state (which is restored) and keeps the fastest.
         Returns ms per step of each candidate."""
-        out = np.zeros(6, dtype=np.float64)
+        out = np.zeros(len(_lib.PLAN_NAMES), dtype=np.float64)
         if _lib.lib().leviathan_autotune(self._h, float(dt), int(steps), _p(out)) != 0:
             _lib.check("autotune")
-        names = ["direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"]
-        return {n: float(v) for n, v in zip(names, out) if v == v}
+        return {n: float(v) for n, v in zip(_lib.PLAN_NAMES, out) if v == v}
 
     def close(self) -> None:
         if self._h is not None:

@@ -340,15 +340,17 @@ def ba200k():
     return wl
 
 
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
 @pytest.mark.parametrize("history", ["f32", "u16"])
 @pytest.mark.parametrize("layout", ["node", "slot"])
-def test_binned_equals_direct(ba200k, history, layout):
-    """The binned schedule shares the direct step's per-row arithmetic and summation
-    order: phases and weights agree to FP32 rounding after 40 steps."""
-    say("binned vs direct", history, layout)
+def test_binned_equals_direct(ba200k, history, layout, schedule):
+    """The two-phase schedules share the direct step's per-row arithmetic and summation
+    order: phases and weights agree to FP32 rounding after 40 steps (the graph has rows
+    and sources beyond one block's capacity)."""
+    say(schedule, "vs direct", history, layout)
     d = make(ba200k, history=history, layout=layout)
-    b = make(ba200k, history=history, layout=layout, schedule="binned")
-    assert b.get_schedule() == ("binned", 8)
+    b = make(ba200k, history=history, layout=layout, schedule=schedule)
+    assert b.get_schedule() == (schedule, 8)
     rd = d.step_n(DT, 40)
     rb = b.step_n(DT, 40)
     assert np.abs(rd - rb).max() < 1e-6
@@ -356,11 +358,14 @@ def test_binned_equals_direct(ba200k, history, layout):
     assert np.abs(b.get_weights() - d.get_weights()).max() < 1e-6
 
 
-@pytest.mark.parametrize("bucket_edges", [0, 2048])
-def test_binned_one_step_consistency(ws1k, bucket_edges):
-    """Binned schedule (default buckets and minimum-size buckets) against the FP64 oracle."""
-    say("binned one-step", bucket_edges)
-    eng = make(ws1k, schedule="binned", bucket_edges=bucket_edges, history="u16", layout="slot")
+@pytest.mark.parametrize("schedule,bucket_edges,ept", [("binned", 0, 8), ("binned", 2048, 8), ("staged", 0, 4),
+                                                       ("staged", 0, 16)])
+def test_binned_one_step_consistency(ws1k, schedule, bucket_edges, ept):
+    """Two-phase schedules (default and minimum-size buckets; staged with the smallest and
+    largest row blocks) against the FP64 oracle."""
+    say(schedule, "one-step", bucket_edges, ept)
+    eng = make(ws1k, schedule=schedule, bucket_edges=bucket_edges, history="u16", layout="slot",
+               edges_per_thread=ept)
     eng.step_n(DT, 33)
     o = oracle_from_engine(eng, ws1k)
     o.step(DT)
@@ -369,9 +374,10 @@ def test_binned_one_step_consistency(ws1k, bucket_edges):
     assert w_rel_err(eng.get_weights(), o.w) < TOL
 
 
-def test_binned_trajectory_vs_golden(ws1k, golden1k):
-    say("binned trajectory")
-    eng = make(ws1k, schedule="binned")
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
+def test_binned_trajectory_vs_golden(ws1k, golden1k, schedule):
+    say(schedule, "trajectory")
+    eng = make(ws1k, schedule=schedule)
     eng.step_n(DT, 10)
     assert theta_rel_err(eng.get_theta(), golden1k["theta_10"]) < TOL
     assert w_rel_err(eng.get_weights(), golden1k["w_10"]) < TOL
@@ -386,7 +392,7 @@ def test_autotune_restores_state_and_switches(ba200k):
     ref.step_n(DT, 5)
     times = a.autotune(DT, steps=3)
     say("autotune ms/step", times)
-    assert set(times) == {"direct/4", "direct/8", "direct/16", "binned/4", "binned/8", "binned/16"}
+    assert set(times) == {f"{k}/{e}" for k in ("direct", "binned", "staged") for e in (4, 8, 16)}
     assert a.step_count == 5
     assert np.array_equal(a.get_theta(), ref.get_theta())
     assert np.array_equal(a.get_weights(), ref.get_weights())
@@ -396,7 +402,8 @@ def test_autotune_restores_state_and_switches(ba200k):
     rr = ref.step_n(DT, 20)
     assert np.abs(ra - rr).max() < 1e-6
     assert theta_rel_err(a.get_theta(), ref.get_theta()) < 1e-6
-    for s, e in [("direct", 4), ("binned", 16), ("direct", 16), ("binned", 4)]:
+    for s, e in [("direct", 4), ("binned", 16), ("staged", 8), ("direct", 16), ("staged", 16), ("binned", 4),
+                 ("staged", 4)]:
         a.set_schedule(s, e)
         a.step_n(DT, 3)
         ref.step_n(DT, 3)
@@ -410,11 +417,12 @@ def test_autotune_restores_state_and_switches(ba200k):
         ref.set_schedule("binned")
 
 
-def test_binned_ensemble_equals_separate():
-    """Ensemble in the binned schedule == separate direct engines."""
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
+def test_binned_ensemble_equals_separate(schedule):
+    """Ensemble in a two-phase schedule == separate direct engines."""
     from leviathan_v1_amd import LeviathanEngine, graph, workloads
 
-    say("binned ensemble")
+    say(schedule, "ensemble")
     sizes = [900, 2500, 1300]
     wls = [workloads.watts_strogatz_workload(n, 20, 0.1, 50, seed=30 + i) for i, n in enumerate(sizes)]
     g, offs = graph.disjoint_union([w.graph for w in wls])
@@ -422,7 +430,7 @@ def test_binned_ensemble_equals_separate():
     g0 = [0.3, 2.0, 3.5]
     ens = LeviathanEngine(g.num_nodes, 50, g.row_ptr, g.col_idx, cat("delays"), cat("weights"), cat("theta"),
                           cat("theta_hat"), cat("omega"), n_nets=3, net_offsets=offs, g0=g0,
-                          controller=[False] * 3, schedule="binned", bucket_edges=4096)
+                          controller=[False] * 3, schedule=schedule, bucket_edges=4096)
     r_ens = ens.step_n(DT, 60)
     th = ens.get_theta()
     for k, w in enumerate(wls):
@@ -432,7 +440,8 @@ def test_binned_ensemble_equals_separate():
         assert theta_rel_err(e.get_theta(), th[offs[k]:offs[k + 1]]) < 1e-6, k
 
 
-@pytest.mark.parametrize("history,schedule", [("f32", "direct"), ("u16", "direct"), ("u16", "binned")])
+@pytest.mark.parametrize("history,schedule", [("f32", "direct"), ("u16", "direct"), ("u16", "binned"),
+                                              ("u16", "staged")])
 def test_mid_size_long_run_r_statistics(history, schedule):
     """WS N=100,000 k=20 p=0.1 with plasticity and homeostasis, 10,000 steps: windowed
     (500-step) mean r and r variance within 5e-3 of the FP64 oracle's run (golden fixture)."""
