@@ -64,7 +64,7 @@ struct DevPlan {
   int ept = 8;
   int4* d_items = nullptr;
   int32_t n_items = 0;
-  int2* d_ranges = nullptr;
+  int4* d_ranges = nullptr;
   bool owns = true;            // binned and staged plans share the row blocks of the direct plan with the same ept
   uint32_t* d_pos = nullptr;   // two-phase plans: [E] value slots in the scatter kernel's write order
   uint16_t* d_rank = nullptr;  // two-phase plans: [E] CSC edge -> index in its source block's write order
@@ -142,6 +142,9 @@ template <class T>
 bool dalloc(LeviathanHandle* h, T** p, size_t count) {
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
   hipError_t e = hipMalloc((void**)p, bytes);
+  // an empty array still gets one zeroed element: the row pass clamps its
+  // loads into each block and reads element 0 of an empty edge array
+  if (e == hipSuccess && count == 0) e = hipMemset(*p, 0, bytes);
   if (e != hipSuccess) {
     set_err("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     *p = nullptr;
@@ -210,9 +213,9 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.ring = h->d_ring;
   a.ring_pitch = h->ring_pitch;
   a.n_slots = h->n_slots;
-  a.num_blocks = pl.n_items;
+  a.num_items = pl.n_items;
   a.xcd_remap = 1;
-  a.blocks = pl.d_items;
+  a.items = pl.d_items;
   a.partials = h->d_partials;
   a.ctrl = h->d_ctrl;
   a.step = h->d_step;
@@ -630,7 +633,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     DevPlan& pl = h->plans[q];
     if (!dalloc(h, &pl.d_items, hp[q].items.size()) || !dalloc(h, &pl.d_ranges, (size_t)n_nets)) return fail();
     if (!hip_ok(hipMemcpy(pl.d_items, hp[q].items.data(), sizeof(int4) * hp[q].items.size(), H2D), "copy plan") ||
-        !hip_ok(hipMemcpy(pl.d_ranges, hp[q].ranges.data(), sizeof(int2) * n_nets, H2D), "copy ranges"))
+        !hip_ok(hipMemcpy(pl.d_ranges, hp[q].ranges.data(), sizeof(int4) * n_nets, H2D), "copy ranges"))
       return fail();
   }
   if (want_binned || want_staged) {

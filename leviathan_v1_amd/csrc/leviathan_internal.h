@@ -15,6 +15,12 @@ constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-r
 
 // Row block flag: a single row whose in-degree exceeds the block capacity.
 constexpr int kBlockLongRow = 1;
+
+// Row-pass work item: {first row, first edge, end edge, net << 10 | rows << 1 | long}.
+__host__ __device__ inline int4 pack_item(int32_t r0, int32_t e0, int32_t e1, int32_t nrows, int32_t net,
+                                          bool is_long) {
+  return make_int4(r0, e0, e1, (net << 10) | (nrows << 1) | (is_long ? kBlockLongRow : 0));
+}
 constexpr int kSerialRow = 32;       // stream blocks: longer rows are filled and summed by a wavefront
 
 // Per-network controller state, read by the step kernel (g) and updated by
@@ -45,10 +51,10 @@ struct StepArgs {
   void* ring;
   int64_t ring_pitch;
   int32_t n_slots;        // S = max_delay + 2
-  int32_t num_blocks;
+  int32_t num_items;        // row blocks (long ones first, then stream ones), one workgroup each
   int32_t xcd_remap;
-  const int4* blocks;     // {row_begin, row_end, net, flags}
-  double2* partials;      // per row block: (sum cos, sum sin) of the new phases
+  const int4* items;        // pack_item layout
+  double2* partials;        // per item: (sum cos, sum sin) of the new phases
   const NetCtrl* ctrl;
   const int64_t* step;    // global step counter (device)
   const float* drive_b;   // NULL: no input drive
@@ -64,7 +70,7 @@ struct FinalizeArgs {
   NetCtrl* ctrl;
   int64_t* step;
   float* r_hist;          // [kRHistSteps][n_nets]
-  const int2* ranges;     // per network: [begin, end) of its partials in the active plan
+  const int4* ranges;     // per network: [begin, end) of its long and of its stream items' partials
   int32_t n_nets;
   int32_t num_blocks;     // partials of the active plan
   float dt;

@@ -40,6 +40,22 @@ namespace lev {
 // source.  The block reads that range with coalesced loads into LDS and picks
 // edge e's value at e0 + lpos[e] there; a long row's range is in CSR order and
 // is read straight.
+struct Item {
+  int32_t r0, e0, nnz, nrows, net;
+  bool is_long;
+};
+
+__device__ __forceinline__ Item unpack_item(int4 v) {
+  Item it;
+  it.r0 = v.x;
+  it.e0 = v.y;
+  it.nnz = v.z - v.y;
+  it.nrows = (v.w >> 1) & 511;
+  it.is_long = (v.w & kBlockLongRow) != 0;
+  it.net = v.w >> 10;
+  return it;
+}
+
 template <class H, int LAYOUT, int EPT, int GATHER>
 __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   constexpr int CAP = kBlock * EPT;
@@ -50,18 +66,21 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   __shared__ double s_red[2][kBlock / 64];
 
   const int tid = threadIdx.x;
-  const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
-  const int4 blk = a.blocks[b];
-  const int32_t r0 = blk.x, r1 = blk.y, net = blk.z;
+  const int32_t b = xcd_block(blockIdx.x, a.num_items, a.xcd_remap);
+  const Item it = unpack_item(a.items[b]);  // carries its edge range: no row_ptr round trip first
+  if (it.nrows == 0) {  // padding item
+    if (threadIdx.x == 0) a.partials[b] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int32_t r0 = it.r0, r1 = it.r0 + it.nrows, net = it.net;
   const int64_t step = *a.step;
   const int32_t S = a.n_slots;
   const int32_t cur = (int32_t)(step % S);
   const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
   const float g = (float)a.ctrl[net].g;
   const float dt = a.dt;
-  const int32_t e0 = a.row_ptr[r0];
-  const int32_t e1 = a.row_ptr[r1];
-  const int32_t nnz = e1 - e0;
+  const int32_t e0 = it.e0;
+  const int32_t nnz = it.nnz;
   const int32_t nrows = r1 - r0;
   const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
   const H* __restrict__ hbuf = reinterpret_cast<const H*>(a.hbuf);
@@ -71,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 
   double my_c = 0.0, my_s = 0.0;
 
-  if (!(blk.w & kBlockLongRow)) {
+  if (!it.is_long) {
     // ---------------- stream mode ----------------
     RowState rs_;
     int32_t rs = 0, re = 0;
@@ -331,10 +350,14 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
     }
   } else {
     for (int net = wave; net < a.n_nets; net += kFinalizeThreads / 64) {
-      const int2 rg = a.ranges[net];
-      const int32_t lo = rg.x, hi = rg.y;
+      const int4 rg = a.ranges[net];
       double c = 0.0, s = 0.0;
-      for (int32_t q = lo + lane; q < hi; q += 64) {
+      for (int32_t q = rg.x + lane; q < rg.y; q += 64) {
+        const double2 p = a.partials[q];
+        c += p.x;
+        s += p.y;
+      }
+      for (int32_t q = rg.z + lane; q < rg.w; q += 64) {
         const double2 p = a.partials[q];
         c += p.x;
         s += p.y;
@@ -376,7 +399,7 @@ __global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, flo
 
 template <class H, int LAYOUT, int GATHER>
 static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
-  const dim3 grid(a.num_blocks), block(kBlock);
+  const dim3 grid(a.num_items), block(kBlock);
   switch (ept) {
     case 4:
       step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
@@ -408,7 +431,7 @@ static hipError_t launch_step_g(int ept, int gather, const StepArgs& a, hipStrea
 }
 
 hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
-  if (a.num_blocks <= 0) return hipSuccess;
+  if (a.num_items <= 0) return hipSuccess;
   if (history == 0) {
     return layout == 0 ? launch_step_g<float, 0>(ept, gather, a, s) : launch_step_g<float, 1>(ept, gather, a, s);
   }

@@ -32,15 +32,17 @@ void parallel_for(int64_t n, F&& f) {
 }  // namespace
 
 bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap, HostPlan& out, std::string& err) {
-  out.items.clear();
-  out.ranges.assign(n_nets, make_int2(0, 0));
+  if (n_nets >= (1 << 21)) {
+    err = "too many networks (at most 2^21 - 1)";
+    return false;
+  }
+  std::vector<std::vector<int4>> longs(n_nets), shorts(n_nets);
   for (int net = 0; net < n_nets; ++net) {
     const int lo = offs[net], hi = offs[net + 1];
-    std::vector<int4> longs, shorts;
     int r = lo;
     while (r < hi) {
       if (deg_of(row_ptr, r) > cap) {
-        longs.push_back(make_int4(r, r + 1, net, kBlockLongRow));
+        longs[net].push_back(pack_item(r, row_ptr[r], row_ptr[r + 1], 1, net, true));
         ++r;
         continue;
       }
@@ -52,14 +54,22 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
         nnz += d;
         ++r;
       }
-      shorts.push_back(make_int4(start, r, net, 0));
+      shorts[net].push_back(pack_item(start, row_ptr[start], row_ptr[r], r - start, net, false));
     }
-    std::stable_sort(longs.begin(), longs.end(),
-                     [&](const int4& a, const int4& b) { return deg_of(row_ptr, a.x) > deg_of(row_ptr, b.x); });
+    std::stable_sort(longs[net].begin(), longs[net].end(),
+                     [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+  }
+  out.items.clear();
+  out.ranges.assign(n_nets, make_int4(0, 0, 0, 0));
+  for (int net = 0; net < n_nets; ++net) {
     out.ranges[net].x = (int)out.items.size();
-    out.items.insert(out.items.end(), longs.begin(), longs.end());
-    out.items.insert(out.items.end(), shorts.begin(), shorts.end());
+    out.items.insert(out.items.end(), longs[net].begin(), longs[net].end());
     out.ranges[net].y = (int)out.items.size();
+  }
+  for (int net = 0; net < n_nets; ++net) {
+    out.ranges[net].z = (int)out.items.size();
+    out.items.insert(out.items.end(), shorts[net].begin(), shorts[net].end());
+    out.ranges[net].w = (int)out.items.size();
   }
   if (out.items.size() > (size_t)INT32_MAX) {
     err = "too many row blocks";
@@ -161,7 +171,7 @@ bool staged_positions(int N, const int* row_ptr, const int* col, const HostPlan&
   std::atomic<bool> ok{true};
   parallel_for((int64_t)plan.items.size(), [&](int64_t q) {
     const int4 it = plan.items[q];
-    const int32_t e0 = row_ptr[it.x], e1 = row_ptr[it.y];
+    const int32_t e0 = it.y, e1 = it.z;
     if (it.w & kBlockLongRow) {
       for (int32_t e = e0; e < e1; ++e) hpos[e] = (uint32_t)e;  // read straight, in CSR order
       return;

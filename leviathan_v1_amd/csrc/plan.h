@@ -9,11 +9,15 @@
 
 namespace lev {
 
-// Work items of one launch, in launch order, and each network's range of them
-// (a network's items are contiguous; its partial sums are reduced over that range).
+// Work items of the row pass (row blocks), in launch order: every network's
+// long rows (heaviest first within a network, so the longest start first),
+// then every network's stream blocks in CSR order.  An item is
+// {r0, e0, e1, net << 10 | nrows << 1 | long} (pack_item), so a workgroup
+// knows its edge range without reading row_ptr first; its partial sums sit at
+// its index.  ranges[net] = {long_begin, long_end, stream_begin, stream_end}.
 struct HostPlan {
   std::vector<int4> items;
-  std::vector<int2> ranges;
+  std::vector<int4> ranges;
 };
 
 // Phase 1 of the two-phase schedules: the graph by source.
@@ -25,8 +29,7 @@ struct HostScatter {
 };
 
 // Row blocks of the direct schedule: runs of <= 256 rows holding <= cap edges,
-// or a single row with more than cap edges (listed first within its network,
-// heaviest first).
+// or a single row with more than cap edges.
 bool build_direct_plan(const int* row_ptr, int n_nets, const int* net_offsets, int cap, HostPlan& out,
                        std::string& err);
 

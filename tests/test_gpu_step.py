@@ -328,6 +328,46 @@ def test_observatory_surface():
 
 
 # ---------------------------------------------------------------------------
+# edge cases of the row blocks: no edges at all, rows without incoming edges,
+# a hub beyond one block's capacity next to isolated rows
+# ---------------------------------------------------------------------------
+
+def _ragged_workload(n, src, dst, seed):
+    from leviathan_v1_amd import graph, workloads
+
+    g = graph.from_edges(n, np.asarray(src, np.int32), np.asarray(dst, np.int32))
+    delays, weights, theta, thh, omega = workloads.init_state(g, 50, seed)
+    return workloads.Workload("ragged", g, 50, delays, weights, theta, thh, omega)
+
+
+@pytest.mark.parametrize("schedule,ept", [("direct", 4), ("direct", 16), ("binned", 8), ("staged", 4),
+                                          ("staged", 16)])
+def test_edgeless_and_ragged_graphs(schedule, ept):
+    """E = 0 (phases advance by omega alone), and a graph whose rows 1200.. have no
+    incoming edges while node 0 is joined to 1..1500 (a row longer than a 4-edge-per-lane
+    block): one step against the FP64 oracle from the engine's state, in every schedule."""
+    say("ragged", schedule, ept)
+    rng = np.random.default_rng(5)
+    a = rng.integers(1, 1200, 3000)
+    b = rng.integers(1, 1200, 3000)
+    keep = a != b
+    cases = [_ragged_workload(500, [], [], 1),
+             _ragged_workload(2000, np.concatenate([np.zeros(1500, int), a[keep]]),
+                              np.concatenate([np.arange(1, 1501), b[keep]]), 2)]
+    for wl in cases:
+        eng = make(wl, schedule=schedule, edges_per_thread=ept, history="u16")
+        r = eng.step_n(DT, 7)
+        assert np.isfinite(r).all()
+        o = oracle_from_engine(eng, wl)
+        o.step(DT)
+        eng.step(DT)
+        assert theta_rel_err(eng.get_theta(), o.theta) < TOL, (wl.E, schedule)
+        if wl.E:
+            assert w_rel_err(eng.get_weights(), o.w) < TOL
+        eng.close()
+
+
+# ---------------------------------------------------------------------------
 # binned (two-phase) schedule
 # ---------------------------------------------------------------------------
 
