@@ -18,6 +18,10 @@
 // The reference needs the kernel boundary because its ring has max_delay+1
 // slots and the write slot aliases the max_delay read slot.
 //
+// Every stream is read and written with plain loads and stores: on the
+// 10^7-node graph the streaming (nontemporal) hint made the row pass no
+// faster (0-7 % slower across runs).
+//
 // Row blocks (built on the host) are either a run of up to 256 consecutive
 // rows holding at most 256*EPT incoming edges in total (stream mode: the
 // edges of the whole run are streamed with consecutive lanes on consecutive
@@ -123,14 +127,14 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) {
         if constexpr (GATHER == 0) {
-          jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
-          dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+          jj[k] = a.col[e0 + le];
+          dd[k] = a.delay[e0 + le];
         } else if constexpr (GATHER == 1) {
-          jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+          jj[k] = (int32_t)a.hpos[e0 + le];
         } else {
-          jj[k] = (int32_t)__builtin_nontemporal_load(a.lpos + e0 + le);
+          jj[k] = (int32_t)a.lpos[e0 + le];
         }
-        ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
+        ww[k] = a.w[e0 + le];
       }
     }
     if constexpr (GATHER == 2) {
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * kBlock + tid;
-        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(__builtin_nontemporal_load(hbuf + e0 + le));
+        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(hbuf[e0 + le]);
       }
     }
     __syncthreads();
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         float c;
         const float wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
         s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
-        __builtin_nontemporal_store(wn, a.w + e0 + le);
+        a.w[e0 + le] = wn;
       }
     }
     __syncthreads();
@@ -225,14 +229,14 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
           if constexpr (GATHER == 0) {
-            jj[k] = __builtin_nontemporal_load(a.col + e0 + le);
-            dd[k] = __builtin_nontemporal_load(a.delay + e0 + le);
+            jj[k] = a.col[e0 + le];
+            dd[k] = a.delay[e0 + le];
           } else if constexpr (GATHER == 1) {
-            jj[k] = (int32_t)__builtin_nontemporal_load(a.hpos + e0 + le);
+            jj[k] = (int32_t)a.hpos[e0 + le];
           } else {
             jj[k] = 0;
           }
-          ww[k] = __builtin_nontemporal_load(a.w + e0 + le);
+          ww[k] = a.w[e0 + le];
         }
       }
 #pragma unroll
@@ -246,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           } else if constexpr (GATHER == 1) {
             hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
           } else {
-            hh[k] = HistCodec<H>::decode(__builtin_nontemporal_load(hbuf + e0 + le));
+            hh[k] = HistCodec<H>::decode(hbuf[e0 + le]);
           }
         }
       }
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           float c;
           const float wn = edge_update(hh[k], th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
           acc.add(c);
-          __builtin_nontemporal_store(wn, a.w + e0 + le);
+          a.w[e0 + le] = wn;
         }
       }
     }
