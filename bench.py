@@ -88,7 +88,7 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
             "edge_steps_per_s": wl.E * steps / el,
             "hbm_gbs": algorithmic_bytes(wl.N, wl.E, history) * steps / el / 1e9,
             "step_kernel_ms": prof["step_kernel_ms"], "finalize_kernel_ms": prof["finalize_kernel_ms"],
-            "scatter_kernel_ms": prof["scatter_kernel_ms"],
+            "scatter_kernel_ms": prof["scatter_kernel_ms"], "regroup_kernel_ms": prof["regroup_kernel_ms"],
             "r_last": float(r[-1]), "invariants_ok": ok, "vram_bytes": eng.get_vram_usage(),
             "row_blocks": eng.num_blocks,
         }
@@ -150,7 +150,7 @@ def load_traffic(workload, history, schedule):
     if not rec:
         return None
     ks = {k: v for k, v in rec["kernels"].items()
-          if "step_kernel" in k or "scatter_kernel" in k or "bucket_kernel" in k}
+          if "step_kernel" in k or "scatter_kernel" in k or "regroup_kernel" in k}
     return sum(v["fetch_bytes"] + v["write_bytes"] for v in ks.values()) or None
 
 
@@ -210,8 +210,9 @@ def main():
     hist = head["history"]
     alg = algorithmic_bytes(wl.N, wl.E, hist)
     # the step's kernels: the fused step kernel (direct) or scatter + step kernels (two-phase schedules)
-    step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"]
-    kernels = "step_kernel" if head["schedule"].startswith("direct") else "scatter_kernel+step_kernel"
+    step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"] + head["regroup_kernel_ms"]
+    kernels = {"direct": "step_kernel", "binned": "scatter_kernel+step_kernel",
+               "staged": "scatter_kernel+regroup_kernel+step_kernel"}[head["schedule"].split("/")[0]]
     achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:

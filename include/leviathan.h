@@ -161,11 +161,11 @@ int64_t leviathan_step_count(LeviathanHandle* handle);
 int leviathan_num_blocks(LeviathanHandle* handle);
 
 /* Runs n eager steps and times them with HIP events on the handle's stream.
- * out[0] = mean duration of the step's main kernel (ms per launch): the fused
- *          step kernel (direct schedule) or the bucket kernel (binned),
+ * out[0] = mean duration of the row pass, the step kernel (ms per launch),
  * out[1] = mean duration of the per-step finalize kernel (ms per launch),
  * out[2] = wall time of the n steps between the first and last event (ms),
- * out[3] = mean duration of the scatter kernel (binned schedule; 0 otherwise).
+ * out[3] = mean duration of the scatter kernel (two-phase schedules; 0 otherwise),
+ * out[4] = mean duration of the regroup kernel (staged schedule; 0 otherwise).
  * Returns 0 or -1. */
 int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* out);
 

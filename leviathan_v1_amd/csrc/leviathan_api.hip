@@ -65,10 +65,15 @@ struct DevPlan {
   int4* d_items = nullptr;
   int32_t n_items = 0;
   int4* d_ranges = nullptr;
+  int32_t xcd_split[lev::kNumXcd + 1] = {};
   bool owns = true;            // binned and staged plans share the row blocks of the direct plan with the same ept
   uint32_t* d_pos = nullptr;   // two-phase plans: [E] value slots in the scatter kernel's write order
   uint16_t* d_rank = nullptr;  // two-phase plans: [E] CSC edge -> index in its source block's write order
   uint16_t* d_lpos = nullptr;  // staged plans: [E] CSR edge -> slot within its block's range
+  int4* d_rg_chunks = nullptr; // staged plans: regroup chunks, ranks and runs
+  uint16_t* d_rg_rank = nullptr;
+  int2* d_rg_runs = nullptr;
+  int32_t n_rg_chunks = 0;
   bool owns_pos = false;
 };
 
@@ -106,7 +111,8 @@ struct LeviathanHandle {
   int32_t* d_csc_ptr = nullptr;
   uint8_t* d_delay_csc = nullptr;
   uint32_t* d_pos = nullptr;
-  void* d_hbuf = nullptr;
+  void* d_hbuf = nullptr;    // two-phase schedules: the scatter kernel's bucketed values
+  void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block
   uint32_t* d_hpos = nullptr;
   uint16_t* d_rank = nullptr;
   int4* d_sources = nullptr;  // phase-1 work items
@@ -167,13 +173,13 @@ void destroy(LeviathanHandle* h) {
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
-                  h->d_pos,     h->d_hbuf,      h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
+                  h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
     if (pl.owns_pos) {
-      void* pp[] = {pl.d_pos, pl.d_rank, pl.d_lpos};
+      void* pp[] = {pl.d_pos, pl.d_rank, pl.d_lpos, pl.d_rg_chunks, pl.d_rg_rank, pl.d_rg_runs};
       for (void* p : pp)
         if (p) (void)hipFree(p);
     }
@@ -214,14 +220,14 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.ring_pitch = h->ring_pitch;
   a.n_slots = h->n_slots;
   a.num_items = pl.n_items;
-  a.xcd_remap = 1;
+  for (int x = 0; x <= lev::kNumXcd; ++x) a.xcd_split[x] = pl.xcd_split[x];
   a.items = pl.d_items;
   a.partials = h->d_partials;
   a.ctrl = h->d_ctrl;
   a.step = h->d_step;
   a.drive_b = h->drive_on ? h->d_drive_b : nullptr;
   a.drive_u = h->d_drive_u;
-  a.hbuf = h->d_hbuf;
+  a.hbuf = pl.kind == LEV_SCHEDULE_STAGED ? h->d_hbuf2 : h->d_hbuf;
   a.hpos = h->d_hpos;
   a.lpos = pl.d_lpos;
   a.dt = dt;
@@ -266,12 +272,24 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
 }
 
 // Enqueue the step kernel(s) of the active plan, without the finalize.
-hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr) {
+// mid / mid2 (optional) are recorded after the scatter and after the regroup.
+hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr) {
   const DevPlan& pl = h->plans[h->active];
   if (pl.kind == LEV_SCHEDULE_DIRECT)
     return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
   hipError_t e = lev::launch_scatter(h->cfg.history, h->cfg.layout, scatter_args(h, pl), h->stream);
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
+  if (e == hipSuccess && pl.kind == LEV_SCHEDULE_STAGED) {
+    lev::RegroupArgs ra{};
+    ra.chunks = pl.d_rg_chunks;
+    ra.rank = pl.d_rg_rank;
+    ra.runs = pl.d_rg_runs;
+    ra.src = h->d_hbuf;
+    ra.dst = h->d_hbuf2;
+    ra.num_chunks = pl.n_rg_chunks;
+    e = lev::launch_regroup(h->cfg.history, ra, h->stream);
+  }
+  if (e == hipSuccess && mid2) e = hipEventRecord(mid2, h->stream);
   if (e == hipSuccess)
     e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1,
                          step_args(h, dt, pl), h->stream);
@@ -531,11 +549,13 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     h->plans[q].kind = LEV_SCHEDULE_DIRECT;
     h->plans[q].ept = epts[q];
     h->plans[q].n_items = (int32_t)hp[q].items.size();
+    for (int x = 0; x <= lev::kNumXcd; ++x) h->plans[q].xcd_split[x] = hp[q].xcd_split[x];
   }
   // two-phase plans: the scatter plan over sources, and per plan the value slots
   lev::HostScatter hsc;
   std::vector<uint32_t> bin_pos, bin_hpos, st_pos[3];
   std::vector<uint16_t> bin_rank, st_rank[3], st_lpos[3];
+  lev::HostRegroup st_rg[3];
   const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
   const bool want_staged = cfg.schedule == LEV_SCHEDULE_STAGED || cfg.schedule == LEV_SCHEDULE_AUTO;
   if (want_binned || want_staged) {
@@ -558,14 +578,19 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       pb.kind = LEV_SCHEDULE_BINNED;
       pb.ept = epts[q];
       pb.n_items = h->plans[q].n_items;
+      for (int x = 0; x <= lev::kNumXcd; ++x) pb.xcd_split[x] = h->plans[q].xcd_split[x];
       pb.owns = false;
     }
   }
   if (want_staged) {
     for (int q = 0; q < 3; ++q) {
-      std::vector<uint32_t> hpos;
+      // the scatter fills buckets of whole row blocks; the regroup moves each value to its staged slot
+      std::vector<uint32_t> hpos, bhpos;
+      std::vector<int32_t> bucket_begin;
       if (!lev::staged_positions(N, row_ptr, col_idx, hp[q], hpos, st_lpos[q], perr) ||
-          !lev::csc_positions(hsc, hpos, st_pos[q], perr)) {
+          !lev::staged_bucket_positions(N, row_ptr, col_idx, hp[q], cfg.bucket_edges, bhpos, bucket_begin, perr) ||
+          !lev::csc_positions(hsc, bhpos, st_pos[q], perr) ||
+          !lev::build_regroup(bhpos, hpos, bucket_begin, st_rg[q], perr)) {
         set_err("%s", perr.c_str());
         return fail();
       }
@@ -575,6 +600,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       ps.kind = LEV_SCHEDULE_STAGED;
       ps.ept = epts[q];
       ps.n_items = h->plans[q].n_items;
+      for (int x = 0; x <= lev::kNumXcd; ++x) ps.xcd_split[x] = h->plans[q].xcd_split[x];
       ps.owns = false;
     }
   }
@@ -639,7 +665,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   if (want_binned || want_staged) {
     const size_t E = (size_t)h->E;
     if (!dalloc(h, &h->d_sources, hsc.sources.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
-        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes))
+        !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes) ||
+        (want_staged && !dalloc(h, (char**)&h->d_hbuf2, E * hist_bytes)))
       return fail();
     ok = hip_ok(hipMemcpy(h->d_sources, hsc.sources.data(), sizeof(int4) * hsc.sources.size(), H2D),
                 "copy sources") &&
@@ -666,10 +693,20 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         ps.d_items = h->plans[q].d_items;
         ps.d_ranges = h->plans[q].d_ranges;
         ps.owns_pos = true;
-        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E)) return fail();
+        const lev::HostRegroup& rg = st_rg[q];
+        ps.n_rg_chunks = (int32_t)rg.chunks.size();
+        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E) ||
+            !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
+            !dalloc(h, &ps.d_rg_runs, rg.runs.size()))
+          return fail();
         if (E && !(hip_ok(hipMemcpy(ps.d_pos, st_pos[q].data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
                    hip_ok(hipMemcpy(ps.d_rank, st_rank[q].data(), sizeof(uint16_t) * E, H2D), "copy ranks") &&
-                   hip_ok(hipMemcpy(ps.d_lpos, st_lpos[q].data(), sizeof(uint16_t) * E, H2D), "copy slots")))
+                   hip_ok(hipMemcpy(ps.d_lpos, st_lpos[q].data(), sizeof(uint16_t) * E, H2D), "copy slots") &&
+                   hip_ok(hipMemcpy(ps.d_rg_chunks, rg.chunks.data(), sizeof(int4) * rg.chunks.size(), H2D),
+                          "copy regroup chunks") &&
+                   hip_ok(hipMemcpy(ps.d_rg_rank, rg.rank.data(), sizeof(uint16_t) * E, H2D), "copy regroup ranks") &&
+                   hip_ok(hipMemcpy(ps.d_rg_runs, rg.runs.data(), sizeof(int2) * rg.runs.size(), H2D),
+                          "copy regroup runs")))
           return fail();
       }
     }
@@ -969,40 +1006,46 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
     return -1;
   }
   const bool binned = h->plans[h->active].kind != LEV_SCHEDULE_DIRECT;
-  // per step: [start, (mid: after scatter), after main, after finalize]
-  std::vector<hipEvent_t> ev((size_t)4 * n, nullptr);
+  // per step: [start, after scatter, after regroup, after the row pass, after finalize]
+  constexpr int K = 5;
+  std::vector<hipEvent_t> ev((size_t)K * n, nullptr);
   bool ok = true;
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) ok = false;
   const FinalizeArgs f = fin_args(h, dt, h->plans[h->active]);
   for (int t = 0; t < n && ok; ++t) {
-    ok = hipEventRecord(ev[4 * t], h->stream) == hipSuccess &&
-         enqueue_main(h, dt, binned ? ev[4 * t + 1] : nullptr) == hipSuccess &&
-         hipEventRecord(ev[4 * t + 2], h->stream) == hipSuccess &&
-         lev::launch_finalize(f, h->stream) == hipSuccess && hipEventRecord(ev[4 * t + 3], h->stream) == hipSuccess;
+    hipEvent_t* e = &ev[(size_t)K * t];
+    ok = hipEventRecord(e[0], h->stream) == hipSuccess &&
+         enqueue_main(h, dt, binned ? e[1] : nullptr, binned ? e[2] : nullptr) == hipSuccess &&
+         hipEventRecord(e[3], h->stream) == hipSuccess && lev::launch_finalize(f, h->stream) == hipSuccess &&
+         hipEventRecord(e[4], h->stream) == hipSuccess;
   }
   ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
   if (ok) {
-    double kmain = 0.0, kfin = 0.0, kscat = 0.0;
+    double kmain = 0.0, kfin = 0.0, kscat = 0.0, kreg = 0.0;
     for (int t = 0; t < n; ++t) {
-      float a = 0.f, b = 0.f, c = 0.f;
+      hipEvent_t* e = &ev[(size_t)K * t];
+      float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
       if (binned) {
-        (void)hipEventElapsedTime(&c, ev[4 * t], ev[4 * t + 1]);
-        (void)hipEventElapsedTime(&a, ev[4 * t + 1], ev[4 * t + 2]);
+        (void)hipEventElapsedTime(&c, e[0], e[1]);
+        (void)hipEventElapsedTime(&d, e[1], e[2]);
+        (void)hipEventElapsedTime(&a, e[2], e[3]);
       } else {
-        (void)hipEventElapsedTime(&a, ev[4 * t], ev[4 * t + 2]);
+        (void)hipEventElapsedTime(&a, e[0], e[3]);
       }
-      (void)hipEventElapsedTime(&b, ev[4 * t + 2], ev[4 * t + 3]);
+      (void)hipEventElapsedTime(&b, e[3], e[4]);
       kmain += a;
       kfin += b;
       kscat += c;
+      kreg += d;
     }
     float wall = 0.f;
-    (void)hipEventElapsedTime(&wall, ev[0], ev[4 * n - 1]);
+    (void)hipEventElapsedTime(&wall, ev[0], ev[(size_t)K * n - 1]);
     out[0] = kmain / n;
     out[1] = kfin / n;
     out[2] = wall;
     out[3] = kscat / n;
+    out[4] = kreg / n;
     h->step += n;
   } else {
     set_err("profile_steps: a HIP call failed");

@@ -22,6 +22,9 @@ __host__ __device__ inline int4 pack_item(int32_t r0, int32_t e0, int32_t e1, in
   return make_int4(r0, e0, e1, (net << 10) | (nrows << 1) | (is_long ? kBlockLongRow : 0));
 }
 constexpr int kSerialRow = 32;       // stream blocks: longer rows are filled and summed by a wavefront
+// Estimated row-pass cost of a block (ns of workgroup lifetime), for balancing the XCDs.
+constexpr double kCostBlock = 9860.0, kCostEdge = 7.42, kCostRow = 11.9, kCostHeavyRow = 300.0;
+constexpr double kCostLong = 20900.0, kCostLongEdge = 5.59;
 
 // Per-network controller state, read by the step kernel (g) and updated by
 // the finalize kernel (r, g) in FP64.
@@ -52,7 +55,8 @@ struct StepArgs {
   int64_t ring_pitch;
   int32_t n_slots;        // S = max_delay + 2
   int32_t num_items;        // row blocks (long ones first, then stream ones), one workgroup each
-  int32_t xcd_remap;
+  // items [xcd_split[x], xcd_split[x+1]) run on XCD x: workgroup 8 * j + x takes item xcd_split[x] + j
+  int32_t xcd_split[kNumXcd + 1];
   const int4* items;        // pack_item layout
   double2* partials;        // per item: (sum cos, sum sin) of the new phases
   const NetCtrl* ctrl;
@@ -78,6 +82,19 @@ struct FinalizeArgs {
 
 // Two-phase (binned) step: see leviathan_binned.hip.
 constexpr int kScatterEpt = 8;      // phase 1: out-edges per lane in a source block
+
+// Regroup pass of the staged schedule (leviathan_binned.hip).
+constexpr int kRegroupChunk = 4096;  // bucket slots per regroup workgroup
+constexpr int kRegroupRuns = 1024;   // runs per chunk
+struct RegroupArgs {
+  const int4* chunks;        // {first slot, count, first run, run count}
+  const uint16_t* rank;      // [E] bucket slot -> index in its chunk's staged order
+  const int2* runs;          // {first index in the chunk's staged order, staged slot}
+  const void* src;           // bucketed values
+  void* dst;                 // staged values
+  int32_t num_chunks;
+};
+hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
 
 struct ScatterArgs {
   const int32_t* csc_ptr;   // [N+1] out-edge offsets per source

@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "leviathan_internal.h"
 #include "step_kernel.h"
 
@@ -70,7 +72,9 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   __shared__ double s_red[2][kBlock / 64];
 
   const int tid = threadIdx.x;
-  const int32_t b = xcd_block(blockIdx.x, a.num_items, a.xcd_remap);
+  const int32_t x = blockIdx.x % kNumXcd;
+  const int32_t b = a.xcd_split[x] + (int32_t)(blockIdx.x / kNumXcd);
+  if (b >= a.xcd_split[x + 1]) return;  // this XCD's run is shorter than the longest
   const Item it = unpack_item(a.items[b]);  // carries its edge range: no row_ptr round trip first
   if (it.nrows == 0) {  // padding item
     if (threadIdx.x == 0) a.partials[b] = make_double2(0.0, 0.0);
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
     for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
       const int src = __ffsll((unsigned long long)m) - 1;
-      const int32_t hs = __shfl(rs, src, 64), he = __shfl(re, src, 64);
+      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
       for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
     }
 
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     }
     for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
       const int src = __ffsll((unsigned long long)m) - 1;
-      const int32_t hs = __shfl(rs, src, 64), he = __shfl(re, src, 64);
+      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
       CompSum acc;
       for (int32_t q = hs + lane; q < he; q += 64) acc.add(s_contrib[q]);
       const double tot = wave_sum((double)acc.s + (double)acc.c);
@@ -403,7 +407,9 @@ __global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, flo
 
 template <class H, int LAYOUT, int GATHER>
 static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
-  const dim3 grid(a.num_items), block(kBlock);
+  int32_t longest = 0;
+  for (int x = 0; x < kNumXcd; ++x) longest = std::max(longest, a.xcd_split[x + 1] - a.xcd_split[x]);
+  const dim3 grid((unsigned)(kNumXcd * longest)), block(kBlock);
   switch (ept) {
     case 4:
       step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);

@@ -75,6 +75,32 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
     err = "too many row blocks";
     return false;
   }
+  // XCD runs of equal estimated cost.  Cost of a block in ns of workgroup
+  // lifetime, fitted to per-block timestamps of the row pass on the
+  // Barabasi-Albert 10^7 graph: rows above kSerialRow edges (summed by their
+  // wavefront) and long rows (strided by the whole workgroup) cost extra.
+  std::vector<double> cum(out.items.size() + 1, 0.0);
+  for (size_t q = 0; q < out.items.size(); ++q) {
+    const int4 it = out.items[q];
+    const int32_t nrows = (it.w >> 1) & 511, nnz = it.z - it.y;
+    double c;
+    if (it.w & kBlockLongRow) {
+      c = kCostLong + kCostLongEdge * nnz;
+    } else {
+      int heavy = 0;
+      for (int32_t r = it.x; r < it.x + nrows; ++r) heavy += deg_of(row_ptr, r) > kSerialRow;
+      c = kCostBlock + kCostEdge * nnz + kCostRow * nrows + kCostHeavyRow * heavy;
+    }
+    cum[q + 1] = cum[q] + c;
+  }
+  out.xcd_split.assign(kNumXcd + 1, 0);
+  const double total = cum.back();
+  size_t q = 0;
+  for (int x = 1; x < kNumXcd; ++x) {
+    while (q < out.items.size() && cum[q + 1] <= total * x / kNumXcd) ++q;
+    out.xcd_split[x] = (int32_t)q;
+  }
+  out.xcd_split[kNumXcd] = (int32_t)out.items.size();
   return true;
 }
 
@@ -188,6 +214,7 @@ bool staged_positions(int N, const int* row_ptr, const int* col, const HostPlan&
 }
 
 static_assert(kBlock * kScatterEpt <= 65536, "scatter ranks are 16-bit");
+static_assert(kRegroupChunk <= 65536, "regroup ranks are 16-bit");
 
 void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std::vector<uint16_t>& rank) {
   rank.assign(pos.size(), 0);
@@ -205,6 +232,101 @@ void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std:
       pos[c0 + k] = (uint32_t)(key[k] >> 32);
     }
   });
+}
+
+bool staged_bucket_positions(int N, const int* row_ptr, const int* col, const HostPlan& plan, int bucket_edges,
+                             std::vector<uint32_t>& hpos, std::vector<int32_t>& bucket_begin, std::string& err) {
+  const int64_t E = row_ptr[N];
+  if (bucket_edges < 1024) {
+    err = "bucket_edges must be at least 1024";
+    return false;
+  }
+  // the plan's blocks in row order, each its CSR edge range [e0, e1)
+  std::vector<int2> blocks;
+  blocks.reserve(plan.items.size());
+  for (const int4& it : plan.items)
+    if (((it.w >> 1) & 511) > 0) blocks.push_back(make_int2(it.y, it.z));
+  std::sort(blocks.begin(), blocks.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
+  bucket_begin.assign(1, 0);
+  int64_t filled = 0, at = 0;
+  for (const int2& b : blocks) {
+    if (b.x != at) {
+      err = "internal error: row blocks do not tile the edges";
+      return false;
+    }
+    const int64_t n = b.y - b.x;
+    if (filled > 0 && filled + n > bucket_edges) {
+      bucket_begin.push_back((int32_t)at);
+      filled = 0;
+    }
+    filled += n;
+    at = b.y;
+  }
+  if (at != E) {
+    err = "internal error: row blocks do not tile the edges";
+    return false;
+  }
+  if (bucket_begin.back() != (int32_t)E) bucket_begin.push_back((int32_t)E);
+  hpos.assign((size_t)E, 0);
+  parallel_for((int64_t)bucket_begin.size() - 1, [&](int64_t q) {
+    rank_by_source(col, bucket_begin[q], bucket_begin[q + 1] - bucket_begin[q], hpos.data());
+  });
+  return true;
+}
+
+bool build_regroup(const std::vector<uint32_t>& bucket_hpos, const std::vector<uint32_t>& staged_hpos,
+                   const std::vector<int32_t>& bucket_begin, HostRegroup& out, std::string& err) {
+  const size_t E = bucket_hpos.size();
+  // dst[i]: staged slot of the value in bucket slot i
+  std::vector<uint32_t> dst(E);
+  for (size_t e = 0; e < E; ++e) dst[bucket_hpos[e]] = staged_hpos[e];
+  out.rank.assign(E, 0);
+  const int64_t nb = (int64_t)bucket_begin.size() - 1;
+  std::vector<std::vector<int4>> chunks(nb);
+  std::vector<std::vector<int2>> runs(nb);
+  std::atomic<bool> ok{true};
+  parallel_for(nb, [&](int64_t q) {
+    std::vector<uint64_t> key;
+    int64_t c0 = bucket_begin[q];
+    const int64_t b0 = bucket_begin[q], b1 = bucket_begin[q + 1];
+    while (c0 < b1) {
+      // the longest chunk from c0 within the slot and run limits
+      int64_t n = std::min<int64_t>(kRegroupChunk, b1 - c0);
+      for (;;) {
+        key.resize((size_t)n);
+        for (int64_t l = 0; l < n; ++l) key[l] = ((uint64_t)dst[c0 + l] << 32) | (uint64_t)l;
+        std::sort(key.begin(), key.end());
+        int64_t nr = 1;
+        for (int64_t k = 1; k < n; ++k) nr += (key[k] >> 32) != (key[k - 1] >> 32) + 1;
+        if (nr <= kRegroupRuns) break;
+        n = std::max<int64_t>(1, n * kRegroupRuns / (nr + 1));
+      }
+      const int32_t run0 = (int32_t)runs[q].size();
+      for (int64_t k = 0; k < n; ++k) {
+        const uint32_t d = (uint32_t)(key[k] >> 32);
+        if (k == 0 || d != (uint32_t)(key[k - 1] >> 32) + 1) runs[q].push_back(make_int2((int)k, (int)d));
+        out.rank[c0 + (key[k] & 0xffffffffu)] = (uint16_t)k;
+        if (d < (uint32_t)b0 || d >= (uint32_t)b1) ok.store(false);
+      }
+      chunks[q].push_back(make_int4((int)c0, (int)n, run0, (int)runs[q].size() - run0));
+      c0 += n;
+    }
+  });
+  if (!ok.load()) {
+    err = "internal error: a staged slot lies outside its bucket";
+    return false;
+  }
+  out.chunks.clear();
+  out.runs.clear();
+  for (int64_t q = 0; q < nb; ++q) {
+    const int32_t base = (int32_t)out.runs.size();
+    for (int4 c : chunks[q]) {
+      c.z += base;
+      out.chunks.push_back(c);
+    }
+    out.runs.insert(out.runs.end(), runs[q].begin(), runs[q].end());
+  }
+  return true;
 }
 
 bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std::vector<uint32_t>& pos,

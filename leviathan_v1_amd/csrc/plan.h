@@ -15,9 +15,13 @@ namespace lev {
 // {r0, e0, e1, net << 10 | nrows << 1 | long} (pack_item), so a workgroup
 // knows its edge range without reading row_ptr first; its partial sums sit at
 // its index.  ranges[net] = {long_begin, long_end, stream_begin, stream_end}.
+//
+// xcd_split[0..8] cuts the items into 8 contiguous runs of about equal
+// estimated cost, one per XCD (workgroups are dealt to the XCDs round-robin).
 struct HostPlan {
   std::vector<int4> items;
   std::vector<int4> ranges;
+  std::vector<int32_t> xcd_split;
 };
 
 // Phase 1 of the two-phase schedules: the graph by source.
@@ -50,6 +54,29 @@ bool bucket_positions(int N, const int* row_ptr, const int* col_idx, int bucket_
 // block's first edge (stream blocks only).
 bool staged_positions(int N, const int* row_ptr, const int* col_idx, const HostPlan& plan,
                       std::vector<uint32_t>& hpos, std::vector<uint16_t>& lpos, std::string& err);
+
+// Staged schedule, phase-1 target: buckets are runs of consecutive row blocks
+// of `plan` (in row order) with <= bucket_edges edges, each occupying its CSR
+// edge range, values ordered by (source, CSR index) inside the bucket.
+// hpos[e] = bucket slot of CSR edge e; bucket_begin lists each bucket's first
+// slot (and E at the end).
+bool staged_bucket_positions(int N, const int* row_ptr, const int* col_idx, const HostPlan& plan, int bucket_edges,
+                             std::vector<uint32_t>& hpos, std::vector<int32_t>& bucket_begin, std::string& err);
+
+// The regroup pass moves every value from its bucket slot to its staged slot.
+// A bucket is cut into chunks of <= kRegroupChunk consecutive slots holding
+// <= kRegroupRuns runs; a chunk's values for one row block are one run of
+// consecutive staged slots (the staged region keeps source order), so the
+// chunk is permuted in LDS into staged-slot order (rank) and leaves run by
+// run.  chunks[k] = {first slot, count, first run, run count}; runs[r] =
+// {first index in the chunk's staged order, its staged slot}.
+struct HostRegroup {
+  std::vector<int4> chunks;
+  std::vector<uint16_t> rank;  // [E] per bucket slot
+  std::vector<int2> runs;
+};
+bool build_regroup(const std::vector<uint32_t>& bucket_hpos, const std::vector<uint32_t>& staged_hpos,
+                   const std::vector<int32_t>& bucket_begin, HostRegroup& out, std::string& err);
 
 // pos[c] = hpos[CSR edge of CSC index c], checked to be a permutation of [0, E).
 bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std::vector<uint32_t>& pos,

@@ -61,16 +61,30 @@ __device__ __forceinline__ int64_t ring_index(int64_t pitch, int32_t node, int32
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// One DPP move of a double (both halves) within rows of 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// FP64 sum over the 64 lanes of a wavefront (all lanes active), in a fixed
+// order, returned to every lane.  Data moves by DPP within each row of 16
+// lanes (lane xor 1, xor 2, then rotations by 4 and 8: lane 0 of a row ends
+// with its row's sum) and by readlane across the four rows, with no LDS
+// round trips.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x124>(v);  // row_ror:4
+  v += dpp_d<0x128>(v);  // row_ror:8
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 // Neumaier's compensated summation (Neumaier 1974), one term at a time.

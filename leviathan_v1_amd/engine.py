@@ -283,11 +283,11 @@ class LeviathanEngine:
 
     def profile_steps(self, dt: float, n: int) -> dict:
         """Times n eager steps with HIP events on the engine's stream (ms per launch)."""
-        out = np.zeros(4, dtype=np.float64)
+        out = np.zeros(5, dtype=np.float64)
         if _lib.lib().leviathan_profile_steps(self._h, float(dt), int(n), _p(out)) != 0:
             _lib.check("profile_steps")
         return {"step_kernel_ms": float(out[0]), "finalize_kernel_ms": float(out[1]), "wall_ms": float(out[2]),
-                "scatter_kernel_ms": float(out[3])}
+                "scatter_kernel_ms": float(out[3]), "regroup_kernel_ms": float(out[4])}
 
     def set_schedule(self, schedule: str, edges_per_thread: int = 8) -> None:
         if _lib.lib().leviathan_set_schedule(self._h, _lib.SCHEDULE[schedule], int(edges_per_thread)) != 0:

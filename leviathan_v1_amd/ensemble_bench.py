@@ -77,7 +77,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
                                f"K=linspace(0,4), controller off", "N": N, "E": E, "history": history,
                    "schedule": chosen, "graph_chunk": 100},
         "autotune_ms_per_step": tuned,
-        "step_kernel_ms": prof["step_kernel_ms"] + prof["scatter_kernel_ms"],
+        "step_kernel_ms": prof["step_kernel_ms"] + prof["scatter_kernel_ms"] + prof["regroup_kernel_ms"],
         "sequential_plain": {"value": sequential, "unit": "edge-steps/s", "steps": seq_steps,
                              "ms_per_ensemble_step": 1e3 * el_seq / seq_steps},
         "speedup_vs_sequential": batched / sequential,
