@@ -43,24 +43,31 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   __shared__ H s_val[CAP];  // the block's values in slot order
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
-  const int4 blk = a.blocks[b];
-  const int32_t s0 = blk.x, s1 = blk.y;
+  const int4 blk = a.blocks[b];  // carries its out-edge range: no csc_ptr round trip first
+  const int32_t s0 = blk.x, c0 = blk.y, nnz = blk.z - blk.y;
   const int64_t step = *a.step;
   const int32_t S = a.n_slots;
   const int32_t cur = (int32_t)(step % S);
-  const int32_t c0 = a.csc_ptr[s0];
-  const int32_t c1 = a.csc_ptr[s1];
-  const int32_t nnz = c1 - c0;
   const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
   H* __restrict__ hbuf = reinterpret_cast<H*>(a.hbuf);
   const int64_t pitch = a.ring_pitch;
 
   if (!(blk.w & kBlockLongRow)) {
-    const int32_t nsrc = s1 - s0;
+    const int32_t nsrc = (blk.w >> 1) & 511;
+    int32_t rs = 0, re = 0;
     if (tid < nsrc) {
-      const int32_t rs = a.csc_ptr[s0 + tid] - c0;
-      const int32_t re = a.csc_ptr[s0 + tid + 1] - c0;
+      rs = a.csc_ptr[s0 + tid] - c0;
+      re = a.csc_ptr[s0 + tid + 1] - c0;
+    }
+    // a source with more than kSerialRow out-edges is mapped by its whole wavefront
+    const bool heavy = tid < nsrc && re - rs > kSerialRow;
+    if (tid < nsrc && !heavy)
       for (int32_t q = rs; q < re; ++q) s_esrc[q] = (uint8_t)tid;
+    const int lane = tid & 63;
+    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+      for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
     }
     __syncthreads();
     int32_t dd[EPT];

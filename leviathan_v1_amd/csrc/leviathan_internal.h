@@ -81,7 +81,7 @@ struct FinalizeArgs {
 };
 
 // Two-phase (binned) step: see leviathan_binned.hip.
-constexpr int kScatterEpt = 8;      // phase 1: out-edges per lane in a source block
+constexpr int kScatterEpt = 16;     // phase 1: out-edges per lane in a source block
 
 // Regroup pass of the staged schedule (leviathan_binned.hip).
 constexpr int kRegroupChunk = 4096;  // bucket slots per regroup workgroup
@@ -106,7 +106,7 @@ struct ScatterArgs {
   int32_t n_slots;
   int32_t num_blocks;
   int32_t xcd_remap;
-  const int4* blocks;       // {source_begin, source_end, 0, flags}
+  const int4* blocks;       // pack_item layout: {first source, first out-edge, end out-edge, sources << 1 | long}
   void* hbuf;               // [E] delayed phases, bucketed
   const int64_t* step;
 };

@@ -335,8 +335,20 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
   };
 
   if (a.n_nets == 1) {
+    // eight loads in flight per thread, added in index order
     double c = 0.0, s = 0.0;
-    for (int32_t q = tid; q < a.num_blocks; q += kFinalizeThreads) {
+    int32_t q = tid;
+    for (; q + 7 * kFinalizeThreads < a.num_blocks; q += 8 * kFinalizeThreads) {
+      double2 p[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) p[k] = a.partials[q + k * kFinalizeThreads];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        c += p[k].x;
+        s += p[k].y;
+      }
+    }
+    for (; q < a.num_blocks; q += kFinalizeThreads) {
       const double2 p = a.partials[q];
       c += p.x;
       s += p.y;

@@ -131,7 +131,7 @@ bool build_scatter_plan(int N, const int* row_ptr, const int* col, const uint8_t
   while (s < N) {
     const int d = out.csc_ptr[s + 1] - out.csc_ptr[s];
     if (d > cap_src) {
-      longs.push_back(make_int4(s, s + 1, 0, kBlockLongRow));
+      longs.push_back(pack_item(s, out.csc_ptr[s], out.csc_ptr[s + 1], 1, 0, true));
       ++s;
       continue;
     }
@@ -143,7 +143,7 @@ bool build_scatter_plan(int N, const int* row_ptr, const int* col, const uint8_t
       nnz += dd;
       ++s;
     }
-    shorts.push_back(make_int4(start, s, 0, 0));
+    shorts.push_back(pack_item(start, out.csc_ptr[start], out.csc_ptr[s], s - start, 0, false));
   }
   out.sources.insert(out.sources.end(), longs.begin(), longs.end());
   out.sources.insert(out.sources.end(), shorts.begin(), shorts.end());
@@ -221,7 +221,7 @@ void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std:
   parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {
     const int4 it = sc.sources[q];
     if (it.w & kBlockLongRow) return;
-    const int32_t c0 = sc.csc_ptr[it.x], c1 = sc.csc_ptr[it.y];
+    const int32_t c0 = it.y, c1 = it.z;
     const int32_t n = c1 - c0;
     std::vector<uint64_t> key((size_t)n);
     for (int32_t l = 0; l < n; ++l) key[l] = ((uint64_t)pos[c0 + l] << 32) | (uint32_t)l;

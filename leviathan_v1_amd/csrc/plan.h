@@ -26,7 +26,7 @@ struct HostPlan {
 
 // Phase 1 of the two-phase schedules: the graph by source.
 struct HostScatter {
-  std::vector<int4> sources;        // work items {source_begin, source_end, 0, flags}
+  std::vector<int4> sources;        // work items, pack_item layout {s0, c0, c1, nsrc << 1 | long}
   std::vector<int32_t> csc_ptr;     // [N+1] out-edge offsets
   std::vector<uint8_t> delay_csc;   // [E] delays in CSC order
   std::vector<uint32_t> csc_edge;   // [E] CSC index -> CSR edge
