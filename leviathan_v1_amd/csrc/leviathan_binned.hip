@@ -114,88 +114,102 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   }
 }
 
-// Regroup pass of the staged schedule: one chunk of a bucket per workgroup.
-// The chunk's values are placed in LDS in staged-slot order (host rank), each
-// position learns its run from a block-wide scan of the run heads, and the
-// values leave run by run: consecutive lanes write consecutive staged slots.
+// Regroup pass of the staged schedule: kRegroupChunksPerWg consecutive chunks
+// of the buckets per workgroup.  The loads of all its chunks are issued
+// first (one memory round trip for the group); then, chunk by chunk, the
+// values are placed in LDS in staged-slot order (host rank), each position
+// learns its run from a block-wide scan of the run heads, and the values
+// leave run by run: consecutive lanes write consecutive staged slots.
 template <class H>
 __global__ __launch_bounds__(kBlock) void regroup_kernel(RegroupArgs a) {
   constexpr int PER = kRegroupChunk / kBlock;
+  constexpr int G = kRegroupChunksPerWg;
   __shared__ H s_val[kRegroupChunk];
   __shared__ uint16_t s_run[kRegroupChunk];  // run heads, then each position's run
   __shared__ int2 s_runs[kRegroupRuns];
   __shared__ int32_t s_wsum[kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int4 ch = a.chunks[blockIdx.x];
-  const int32_t c0 = ch.x, n = ch.y, r0 = ch.z, nr = ch.w;
   const H* __restrict__ src = reinterpret_cast<const H*>(a.src);
   H* __restrict__ dst = reinterpret_cast<H*>(a.dst);
+  const int32_t k0 = blockIdx.x * G;
+  const int32_t ng = min(G, a.num_chunks - k0);
 
-  H v[PER];
-  int32_t rk[PER];
+  int4 ch[G];
+  H v[G][PER];
+  int32_t rk[G][PER];
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int32_t l = k * kBlock + tid;
-    if (l < n) {
-      v[k] = src[c0 + l];
-      rk[k] = a.rank[c0 + l];
+  for (int g = 0; g < G; ++g) {
+    ch[g] = g < ng ? a.chunks[k0 + g] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < ch[g].y) {
+        v[g][k] = src[ch[g].x + l];
+        rk[g][k] = a.rank[ch[g].x + l];
+      }
     }
   }
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int32_t l = k * kBlock + tid;
-    if (l < n) s_run[l] = 0;
-  }
-  for (int32_t r = tid; r < nr; r += kBlock) s_runs[r] = a.runs[r0 + r];
-  __syncthreads();
-  for (int32_t r = tid; r < nr; r += kBlock) s_run[s_runs[r].x] = 1;
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int32_t l = k * kBlock + tid;
-    if (l < n) s_val[rk[k]] = v[k];
-  }
-  __syncthreads();
 
-  // exclusive scan of the heads: lane tid owns positions [PER * tid, PER * tid + PER)
-  int32_t cnt = 0;
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int32_t l = PER * tid + j;
-    cnt += (l < n) ? (int32_t)s_run[l] : 0;
-  }
-  int32_t incl = cnt;
+  for (int g = 0; g < G; ++g) {
+    const int32_t n = ch[g].y, r0 = ch[g].z, nr = ch[g].w;
+    if (g > 0) __syncthreads();  // the previous chunk's values have left the LDS
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int32_t t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
-  }
-  if (lane == 63) s_wsum[wave] = incl;
-  __syncthreads();
-  int32_t run = incl - cnt;
-  for (int w = 0; w < wave; ++w) run += s_wsum[w];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int32_t l = PER * tid + j;
-    if (l < n) {
-      run += s_run[l];
-      s_run[l] = (uint16_t)(run - 1);
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) s_run[l] = 0;
     }
-  }
-  __syncthreads();
+    for (int32_t r = tid; r < nr; r += kBlock) s_runs[r] = a.runs[r0 + r];
+    __syncthreads();
+    for (int32_t r = tid; r < nr; r += kBlock) s_run[s_runs[r].x] = 1;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) s_val[rk[g][k]] = v[g][k];
+    }
+    __syncthreads();
+
+    // exclusive scan of the heads: lane tid owns positions [PER * tid, PER * tid + PER)
+    int32_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t l = PER * tid + j;
+      cnt += (l < n) ? (int32_t)s_run[l] : 0;
+    }
+    int32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - cnt;
+    for (int w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t l = PER * tid + j;
+      if (l < n) {
+        run += s_run[l];
+        s_run[l] = (uint16_t)(run - 1);
+      }
+    }
+    __syncthreads();
 
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int32_t l = k * kBlock + tid;
-    if (l < n) {
-      const int2 rr = s_runs[s_run[l]];
-      dst[rr.y + (l - rr.x)] = s_val[l];
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) {
+        const int2 rr = s_runs[s_run[l]];
+        dst[rr.y + (l - rr.x)] = s_val[l];
+      }
     }
   }
 }
 
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
   if (a.num_chunks <= 0) return hipSuccess;
-  const dim3 grid(a.num_chunks), block(kBlock);
+  const dim3 grid((a.num_chunks + kRegroupChunksPerWg - 1) / kRegroupChunksPerWg), block(kBlock);
   if (history == 0)
     regroup_kernel<float><<<grid, block, 0, s>>>(a);
   else

@@ -86,6 +86,7 @@ constexpr int kScatterEpt = 16;     // phase 1: out-edges per lane in a source b
 // Regroup pass of the staged schedule (leviathan_binned.hip).
 constexpr int kRegroupChunk = 4096;  // bucket slots per regroup workgroup
 constexpr int kRegroupRuns = 1024;   // runs per chunk
+constexpr int kRegroupChunksPerWg = 2;  // chunks one regroup workgroup loads at once
 struct RegroupArgs {
   const int4* chunks;        // {first slot, count, first run, run count}
   const uint16_t* rank;      // [E] bucket slot -> index in its chunk's staged order
