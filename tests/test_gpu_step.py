@@ -367,6 +367,32 @@ def test_edgeless_and_ragged_graphs(schedule, ept):
         eng.close()
 
 
+@pytest.mark.parametrize("history", ["u16", "f32"])
+def test_staged_regroup_many_blocks_per_bucket(history):
+    """A sparse graph (1.2M nodes, 600k random undirected edges): about 2,000 row blocks
+    per 512k-edge bucket, so regroup chunks are cut at the run limit; one step of the
+    staged schedule against the FP64 oracle, and the same step as the direct schedule."""
+    say("staged regroup, sparse", history)
+    rng = np.random.default_rng(9)
+    n = 1_200_000
+    a = rng.integers(0, n, 600_000)
+    b = rng.integers(0, n, 600_000)
+    keep = a != b
+    wl = _ragged_workload(n, a[keep], b[keep], 3)
+    eng = make(wl, schedule="staged", history=history)
+    ref = make(wl, history=history)
+    eng.step_n(DT, 5)
+    ref.step_n(DT, 5)
+    assert np.array_equal(eng.get_weights(), ref.get_weights())
+    o = oracle_from_engine(eng, wl)
+    o.step(DT)
+    eng.step(DT)
+    assert theta_rel_err(eng.get_theta(), o.theta) < TOL
+    assert w_rel_err(eng.get_weights(), o.w) < TOL
+    eng.close()
+    ref.close()
+
+
 # ---------------------------------------------------------------------------
 # binned (two-phase) schedule
 # ---------------------------------------------------------------------------
