@@ -27,10 +27,21 @@
 #pragma once
 
 #include <math.h>
+#include <stdint.h>
+#include <string.h>
 
 namespace lev {
 
 constexpr float kSinCosBoundedMax = 8.0f;
+
+// x with its sign bit xor-ed with bit 31 of s.
+LEV_HD float xor_sign(float x, uint32_t s) {
+  uint32_t u;
+  memcpy(&u, &x, sizeof u);
+  u ^= s & 0x80000000u;
+  memcpy(&x, &u, sizeof x);
+  return x;
+}
 
 LEV_HD void sincos_bounded(float x, float* s, float* c) {
   constexpr float kTwoOverPi = 0x1.45f306p-1f;   // 2/pi rounded to float
@@ -51,13 +62,11 @@ LEV_HD void sincos_bounded(float x, float* s, float* c) {
   pc = fmaf(r2, pc, C2);
   pc = fmaf(r2, pc, C1);
   const float cr = fmaf(r2, pc, 1.0f);
-  const int q = (int)k;
-  float sn = (q & 1) ? cr : sr;
-  float cs = (q & 1) ? sr : cr;
-  if (q & 2) sn = -sn;
-  if ((q + 1) & 2) cs = -cs;
-  *s = sn;
-  *c = cs;
+  const uint32_t q = (uint32_t)(int)k;
+  const bool odd = (q & 1u) != 0;
+  // quadrant q: sin x = (sin r, cos r, -sin r, -cos r), cos x = (cos r, -sin r, -cos r, sin r)
+  *s = xor_sign(odd ? cr : sr, q << 30);
+  *c = xor_sign(odd ? sr : cr, (q + 1u) << 30);
 }
 
 }  // namespace lev

@@ -13,17 +13,9 @@ namespace lev {
 
 // 2*pi rounded to FP32 (the reference's 2.0f * CUDART_PI_F, leviathan_kernel.cu:102).
 constexpr float kTwoPi = 6.28318548f;
-constexpr float kInvTwoPi = 0.159154943f;
 // uint16 exact-wrap angles: q in [0, 65536) stands for q * 2*pi / 65536 radians.
 constexpr float kU16ToRad = 9.58737992e-05f;  // 2*pi / 65536
 constexpr float kRadToU16 = 10430.3784f;      // 65536 / (2*pi)
-
-// Wrap to [-pi, pi) for the coupling argument.  The reference divides by 2*pi
-// (:146); a multiply by the rounded reciprocal changes only which side of the
-// +-pi seam a value lands on, which sin/cos do not see.
-__device__ __forceinline__ float wrap_pm_pi(float x) {
-  return fmaf(-floorf(fmaf(x, kInvTwoPi, 0.5f)), kTwoPi, x);
-}
 
 // Exactly the reference's per-node wraps (true division): prep's [-pi, pi)
 // (:105) and integrate's [0, 2*pi) (:188).
@@ -140,10 +132,14 @@ __device__ __forceinline__ float row_integrate(const RowState& s, float om, floa
 }
 
 // Hebbian update of one edge (leviathan_kernel.cu:157-163); returns the new
-// weight and writes the coupling contribution w_old * sin(delta).
+// weight and writes the coupling contribution w_old * sin(delta).  The
+// reference wraps delta to [-pi, pi) first (:146); sin and cos are
+// 2*pi-periodic and, with both phases in [0, 2*pi), |delta| < 2*pi, which the
+// bounded routine reduces directly (with pi/2 to 2^-76 rather than a float
+// 2*pi), so the wrap is left out.
 __device__ __forceinline__ float edge_update(float h, float th_i, float gate, float w, float dt, float eta, float lam,
                                              float w_min, float w_max, float* contrib) {
-  const float diff = wrap_pm_pi(h - th_i);
+  const float diff = h - th_i;
   float sn, cs;
   sincos_fast(diff, &sn, &cs);
   *contrib = w * sn;
