@@ -158,6 +158,25 @@ int leviathan_get_history(LeviathanHandle* handle, int delay, float* h_out);
 int leviathan_num_nodes(LeviathanHandle* handle);
 int64_t leviathan_num_edges(LeviathanHandle* handle);
 int64_t leviathan_step_count(LeviathanHandle* handle);
+
+/* Full-state restore (a checkpoint is the getters' output: theta, the
+ * predictor, the weights, every history slot, the FP64 gains and the step
+ * count).  No counterpart in the reference, whose only state setter is
+ * set_theta, which erases the history (leviathan_kernel.cu:445-453).  Restore
+ * in this order: set_step_count, reset_weights, set_theta (which fills every
+ * history slot with theta), set_predictor, set_history for delays 1..D,
+ * set_gains; the engine then continues exactly as the one saved.
+ *   leviathan_set_history writes the value theta(t - delay) seen by a delay
+ *   (1 <= delay <= max_delay) in the next step, rounded to the history format;
+ *   leviathan_get_gains / set_gains move each network's FP64 gain (n_nets
+ *   values; get_gain rounds to float);
+ *   leviathan_set_step_count sets the step counter, which selects the history
+ *   slots. */
+int leviathan_set_predictor(LeviathanHandle* handle, const float* h_theta_hat, const float* h_omega_hat);
+int leviathan_set_history(LeviathanHandle* handle, int delay, const float* h_values);
+int leviathan_get_gains(LeviathanHandle* handle, double* g_out);
+int leviathan_set_gains(LeviathanHandle* handle, const double* g);
+int leviathan_set_step_count(LeviathanHandle* handle, int64_t step);
 int leviathan_num_blocks(LeviathanHandle* handle);
 
 /* Runs n eager steps and times them with HIP events on the handle's stream.

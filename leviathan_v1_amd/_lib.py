@@ -68,6 +68,11 @@ SIGNATURES = {
     "leviathan_get_weights": (c_int, [vp, vp]),
     "leviathan_get_predictor": (c_int, [vp, vp, vp]),
     "leviathan_get_history": (c_int, [vp, c_int, vp]),
+    "leviathan_set_predictor": (c_int, [vp, vp, vp]),
+    "leviathan_set_history": (c_int, [vp, c_int, vp]),
+    "leviathan_get_gains": (c_int, [vp, vp]),
+    "leviathan_set_gains": (c_int, [vp, vp]),
+    "leviathan_set_step_count": (c_int, [vp, c_int64]),
     "leviathan_num_nodes": (c_int, [vp]),
     "leviathan_num_edges": (c_int64, [vp]),
     "leviathan_step_count": (c_int64, [vp]),

@@ -989,6 +989,86 @@ int leviathan_get_history(LeviathanHandle* h, int delay, float* out) {
   return 0;
 }
 
+int leviathan_set_predictor(LeviathanHandle* h, const float* theta_hat, const float* omega_hat) {
+  if (!check_handle(h)) return -1;
+  if (!theta_hat || !omega_hat) {
+    set_err("set_predictor needs theta_hat and omega_hat");
+    return -1;
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(h->d_theta_hat, theta_hat, sizeof(float) * h->N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->d_omega_hat, omega_hat, sizeof(float) * h->N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err("set_predictor: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_set_history(LeviathanHandle* h, int delay, const float* values) {
+  if (!check_handle(h)) return -1;
+  if (delay < 1 || delay > h->max_delay || !values) {
+    set_err("set_history needs values and a delay in [1, %d] (delay 0 is theta: set_theta)", h->max_delay);
+    return -1;
+  }
+  const int32_t slot = (int32_t)(((h->step - delay) % h->n_slots + h->n_slots) % h->n_slots);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(h->d_scratch, values, sizeof(float) * h->N, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = lev::launch_write_slot(h->cfg.history, h->cfg.layout, h->d_ring, h->ring_pitch, slot, h->d_scratch, h->N,
+                               h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_err("set_history: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_get_gains(LeviathanHandle* h, double* g) {
+  if (!check_handle(h)) return -1;
+  if (!g) {
+    set_err("get_gains needs an output array");
+    return -1;
+  }
+  std::vector<NetCtrl> ctrl((size_t)h->n_nets);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(ctrl.data(), h->d_ctrl, sizeof(NetCtrl) * h->n_nets, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("get_gains: %s", hipGetErrorString(e));
+    return -1;
+  }
+  for (int k = 0; k < h->n_nets; ++k) g[k] = ctrl[k].g;
+  return 0;
+}
+
+int leviathan_set_gains(LeviathanHandle* h, const double* g) {
+  if (!check_handle(h)) return -1;
+  if (!g) {
+    set_err("set_gains needs the gains");
+    return -1;
+  }
+  for (int k = 0; k < h->n_nets; ++k)
+    if (!refresh_ctrl_field(h, k, offsetof(NetCtrl, g), g + k, sizeof(double))) return -1;
+  return 0;
+}
+
+int leviathan_set_step_count(LeviathanHandle* h, int64_t step) {
+  if (!check_handle(h)) return -1;
+  if (step < 0) {
+    set_err("step count must be non-negative");
+    return -1;
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(h->d_step, &step, sizeof(int64_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err("set_step_count: %s", hipGetErrorString(e));
+    return -1;
+  }
+  h->step = step;
+  return 0;
+}
+
 int leviathan_num_nodes(LeviathanHandle* h) { return h ? h->N : 0; }
 int64_t leviathan_num_edges(LeviathanHandle* h) { return h ? h->E : 0; }
 int64_t leviathan_step_count(LeviathanHandle* h) { return h ? h->step : 0; }

@@ -119,6 +119,8 @@ hipError_t launch_step(int history, int layout, int ept, int gather, const StepA
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
 hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, int32_t n_slots,
                             const float* theta, int32_t n, hipStream_t s);
+hipError_t launch_write_slot(int history, int layout, void* ring, int64_t pitch, int32_t slot, const float* in,
+                             int32_t n, hipStream_t s);
 hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t pitch, int32_t slot,
                             float* out, int32_t n, hipStream_t s);
 // (leviathan_binned.hip)

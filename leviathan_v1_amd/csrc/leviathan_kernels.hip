@@ -408,6 +408,12 @@ __global__ void fill_ring_kernel(H* ring, int64_t pitch, int32_t n_slots, const 
 }
 
 template <class H, int LAYOUT>
+__global__ void write_slot_kernel(H* ring, int64_t pitch, int32_t slot, const float* in, int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ring[ring_index<LAYOUT>(pitch, i, slot)] = HistCodec<H>::encode(in[i]);
+}
+
+template <class H, int LAYOUT>
 __global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, float* out, int32_t n) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, i, slot)]);
@@ -480,6 +486,24 @@ hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, 
       fill_ring_kernel<uint16_t, 0><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, n_slots, theta, n);
     else
       fill_ring_kernel<uint16_t, 1><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, n_slots, theta, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_write_slot(int history, int layout, void* ring, int64_t pitch, int32_t slot, const float* in,
+                             int32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (history == 0) {
+    if (layout == 0)
+      write_slot_kernel<float, 0><<<grid, block, 0, s>>>((float*)ring, pitch, slot, in, n);
+    else
+      write_slot_kernel<float, 1><<<grid, block, 0, s>>>((float*)ring, pitch, slot, in, n);
+  } else {
+    if (layout == 0)
+      write_slot_kernel<uint16_t, 0><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, slot, in, n);
+    else
+      write_slot_kernel<uint16_t, 1><<<grid, block, 0, s>>>((uint16_t*)ring, pitch, slot, in, n);
   }
   return hipGetLastError();
 }

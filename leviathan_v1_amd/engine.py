@@ -206,6 +206,60 @@ class LeviathanEngine:
             _lib.check("get_history")
         return out
 
+    # ------------------------------------------------------------------ full state
+    def set_predictor(self, theta_hat, omega_hat) -> None:
+        thh = _arr(theta_hat, np.float32, self.N, "theta_hat")
+        omh = _arr(omega_hat, np.float32, self.N, "omega_hat")
+        if _lib.lib().leviathan_set_predictor(self._h, _p(thh), _p(omh)) != 0:
+            _lib.check("set_predictor")
+
+    def set_history(self, delay: int, values) -> None:
+        """theta(t - delay) as the next step will read it, for 1 <= delay <= max_delay."""
+        v = _arr(values, np.float32, self.N, "values")
+        if _lib.lib().leviathan_set_history(self._h, int(delay), _p(v)) != 0:
+            _lib.check("set_history")
+
+    def get_gains(self) -> np.ndarray:
+        """Each network's gain in FP64 (the controller's own precision)."""
+        out = np.empty(self.n_nets, dtype=np.float64)
+        if _lib.lib().leviathan_get_gains(self._h, _p(out)) != 0:
+            _lib.check("get_gains")
+        return out
+
+    def set_gains(self, g) -> None:
+        g = _arr(g, np.float64, self.n_nets, "g")
+        if _lib.lib().leviathan_set_gains(self._h, _p(g)) != 0:
+            _lib.check("set_gains")
+
+    def set_step_count(self, step: int) -> None:
+        if _lib.lib().leviathan_set_step_count(self._h, int(step)) != 0:
+            _lib.check("set_step_count")
+
+    def checkpoint(self) -> dict:
+        """Everything the next steps depend on: phases, predictor, weights, the history seen by
+        delays 1..max_delay, FP64 gains and the step count (see leviathan_v1_amd.checkpoint)."""
+        thh, omh = self.get_predictor()
+        D = self.max_delay
+        hist = np.empty((D, self.N), dtype=np.float32)
+        for d in range(1, D + 1):
+            hist[d - 1] = self.get_history(d)
+        return {"theta": self.get_theta(), "theta_hat": thh, "omega_hat": omh, "weights": self.get_weights(),
+                "history": hist, "gains": self.get_gains(), "step": np.int64(self.step_count),
+                "N": np.int64(self.N), "E": np.int64(self.E), "max_delay": np.int64(D)}
+
+    def restore(self, state: dict) -> None:
+        """Continue from a checkpoint() of an engine with the same graph and delays."""
+        if (int(state["N"]), int(state["E"]), int(state["max_delay"])) != (self.N, self.E, self.max_delay):
+            raise ValueError("checkpoint is of a different graph (N, E or max_delay differ)")
+        self.set_step_count(int(state["step"]))
+        self.reset_weights(state["weights"])
+        self.set_theta(state["theta"])
+        self.set_predictor(state["theta_hat"], state["omega_hat"])
+        hist = np.asarray(state["history"])
+        for d in range(1, self.max_delay + 1):
+            self.set_history(d, hist[d - 1])
+        self.set_gains(state["gains"])
+
     # ------------------------------------------------------------------ reservoir features
     def features_setup(self, nodes, max_rows: int) -> None:
         nodes = _arr(nodes, np.int32, None, "nodes")

@@ -393,6 +393,53 @@ def test_staged_regroup_many_blocks_per_bucket(history):
     ref.close()
 
 
+@pytest.mark.parametrize("history,schedule", [("f32", "direct"), ("u16", "staged"), ("u16", "binned")])
+def test_checkpoint_restore_continues_exactly(ws1k, tmp_path, history, schedule):
+    """A fresh engine restored from a checkpoint file (theta, predictor, weights, every
+    history slot, FP64 gain, step count) continues bit for bit like the engine it came from."""
+    from leviathan_v1_amd import checkpoint
+
+    say("checkpoint", history, schedule)
+    a = make(ws1k, history=history, schedule=schedule)
+    a.step_n(DT, 25)
+    p = tmp_path / "state.npz"
+    g = ws1k.graph
+    checkpoint.save(p, a.checkpoint(), g.row_ptr, g.col_idx, ws1k.delays)
+    ra = a.step_n(DT, 30)
+    b = make(ws1k, history=history, schedule=schedule)
+    b.step_n(DT, 3)  # any prior state is overwritten
+    b.restore(checkpoint.load(p, g.row_ptr, g.col_idx, ws1k.delays))
+    assert b.step_count == 25
+    rb = b.step_n(DT, 30)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(a.get_theta(), b.get_theta())
+    assert np.array_equal(a.get_weights(), b.get_weights())
+    assert np.array_equal(a.get_gains(), b.get_gains())
+
+
+def test_checkpoint_restore_ensemble():
+    """Ensembles: per-network FP64 gains (controller on for one network) survive the restore."""
+    from leviathan_v1_amd import LeviathanEngine, graph, workloads
+
+    say("checkpoint ensemble")
+    wls = [workloads.watts_strogatz_workload(n, 20, 0.1, 50, seed=60 + i) for i, n in enumerate([700, 1500])]
+    g, offs = graph.disjoint_union([w.graph for w in wls])
+    cat = lambda f: np.concatenate([getattr(w, f) for w in wls])
+    args = (g.num_nodes, 50, g.row_ptr, g.col_idx, cat("delays"), cat("weights"), cat("theta"),
+            cat("theta_hat"), cat("omega"))
+    kw = dict(n_nets=2, net_offsets=offs, g0=[0.7, 2.5], controller=[True, False], history="u16")
+    a = LeviathanEngine(*args, **kw)
+    a.step_n(DT, 40)
+    st = a.checkpoint()
+    ra = a.step_n(DT, 20)
+    b = LeviathanEngine(*args, **kw)
+    b.restore(st)
+    rb = b.step_n(DT, 20)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(a.get_gains(), b.get_gains())
+    assert a.get_gains()[0] != 0.7  # the controller moved the first network's gain
+
+
 # ---------------------------------------------------------------------------
 # binned (two-phase) schedule
 # ---------------------------------------------------------------------------
