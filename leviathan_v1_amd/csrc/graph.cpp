@@ -330,4 +330,36 @@ void lev_graph_copy_csr(const lev_graph* g, int32_t* row_ptr, int32_t* col_idx) 
 
 void lev_graph_free(lev_graph* g) { delete g; }
 
+// Relabel the nodes of an incoming-edge CSR: new node i is old node perm[i].
+// Writes the new CSR (row_ptr[n+1], col_idx[E], columns ascending within a
+// row) and emap[E]: the old CSR index of each new entry, to carry per-edge
+// arrays (delays, weights) along.  Returns 0, or -1 if perm is not a
+// permutation of [0, n).
+int lev_graph_permute(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const int32_t* perm,
+                      int32_t* out_row_ptr, int32_t* out_col, int64_t* emap) {
+  std::vector<int32_t> inv((size_t)n, -1);
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t p = perm[i];
+    if (p < 0 || p >= n || inv[p] >= 0) {
+      g_err = "not a permutation";
+      return -1;
+    }
+    inv[p] = i;
+  }
+  out_row_ptr[0] = 0;
+  for (int32_t i = 0; i < n; ++i) out_row_ptr[i + 1] = out_row_ptr[i] + (row_ptr[perm[i] + 1] - row_ptr[perm[i]]);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t o0 = row_ptr[perm[i]], deg = row_ptr[perm[i] + 1] - o0, n0 = out_row_ptr[i];
+    std::vector<std::pair<int32_t, int64_t>> row((size_t)deg);
+    for (int32_t k = 0; k < deg; ++k) row[k] = {inv[col_idx[o0 + k]], (int64_t)o0 + k};
+    std::sort(row.begin(), row.end());
+    for (int32_t k = 0; k < deg; ++k) {
+      out_col[n0 + k] = row[k].first;
+      emap[n0 + k] = row[k].second;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -44,6 +44,8 @@ def _load():
         lib.lev_graph_free.restype = None
         lib.lev_graph_free.argtypes = [vp]
         lib.lev_graph_last_error.restype = ctypes.c_char_p
+        lib.lev_graph_permute.restype = ctypes.c_int
+        lib.lev_graph_permute.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp]
         _lib = lib
     return _lib
 
@@ -99,6 +101,59 @@ def from_edges(n: int, src, dst) -> CSRGraph:
     if src.shape != dst.shape:
         raise ValueError("src and dst must have the same length")
     return _take(_load().lev_graph_from_edges(int(n), int(src.shape[0]), src.ctypes.data, dst.ctypes.data))
+
+
+def save(path, g: CSRGraph, **arrays) -> None:
+    """Write a graph (and optional per-node / per-edge arrays, e.g. delays) as an .npz of plain
+    arrays: row_ptr int32[N+1], col_idx int32[E]."""
+    np.savez(path, row_ptr=g.row_ptr, col_idx=g.col_idx, **arrays)
+
+
+def load(path) -> tuple[CSRGraph, dict]:
+    """Read a graph written by save(); returns (graph, the other arrays).  The CSR is checked:
+    row_ptr starts at 0 and never decreases, and every column is a node."""
+    with np.load(path, allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files}
+    row_ptr = np.ascontiguousarray(arrays.pop("row_ptr"), dtype=np.int32)
+    col_idx = np.ascontiguousarray(arrays.pop("col_idx"), dtype=np.int32)
+    n = row_ptr.shape[0] - 1
+    if (n < 0 or row_ptr[0] != 0 or (np.diff(row_ptr) < 0).any() or row_ptr[-1] != col_idx.shape[0]
+            or (col_idx.size and (col_idx.min() < 0 or col_idx.max() >= n))):
+        raise ValueError(f"{path} does not hold a valid CSR")
+    return CSRGraph(row_ptr, col_idx), arrays
+
+
+def permute(g: CSRGraph, perm) -> tuple[CSRGraph, np.ndarray]:
+    """Relabel nodes (new node i is old node perm[i]); returns the new CSR, columns ascending
+    within each row, and the edge map (old CSR index of each new entry) for per-edge arrays."""
+    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    n = g.num_nodes
+    if perm.shape != (n,):
+        raise ValueError("perm must have one entry per node")
+    row_ptr = np.empty(n + 1, dtype=np.int32)
+    col_idx = np.empty(g.num_edges, dtype=np.int32)
+    emap = np.empty(g.num_edges, dtype=np.int64)
+    if _load().lev_graph_permute(n, g.row_ptr.ctypes.data, g.col_idx.ctypes.data, perm.ctypes.data,
+                                 row_ptr.ctypes.data, col_idx.ctypes.data, emap.ctypes.data) != 0:
+        raise ValueError(_load().lev_graph_last_error().decode())
+    return CSRGraph(row_ptr, col_idx), emap
+
+
+def degree_order(g: CSRGraph) -> np.ndarray:
+    """Nodes by in-degree, highest first (ties by index): hubs get neighbouring labels, so their
+    history lines share cache lines and the hub rows form the first row blocks."""
+    return np.argsort(-g.degrees(), kind="stable").astype(np.int32)
+
+
+def rcm_order(g: CSRGraph) -> np.ndarray:
+    """Reverse Cuthill-McKee order of the (symmetric) graph, from scipy.sparse.csgraph: neighbours
+    get nearby labels, which shortens gathers on graphs with bounded bandwidth."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+    n = g.num_nodes
+    m = csr_matrix((np.ones(g.num_edges, dtype=np.int8), g.col_idx, g.row_ptr), shape=(n, n))
+    return np.ascontiguousarray(reverse_cuthill_mckee(m, symmetric_mode=True), dtype=np.int32)
 
 
 def disjoint_union(graphs) -> tuple[CSRGraph, np.ndarray]:

@@ -47,6 +47,15 @@ class Workload:
                 self.weights, self.theta, self.theta_hat, self.omega)
 
 
+def relabel(wl: Workload, perm, name: str | None = None) -> Workload:
+    """The same workload with node i of the result being node perm[i] of wl: graph, delays,
+    weights and node states carried along (see graph.permute, degree_order, rcm_order)."""
+    perm = np.asarray(perm, dtype=np.int32)
+    g, emap = _graph.permute(wl.graph, perm)
+    return Workload(name or f"{wl.name} relabelled", g, wl.max_delay, wl.delays[emap], wl.weights[emap],
+                    wl.theta[perm], wl.theta_hat[perm], wl.omega[perm], wl.dt, dict(wl.meta, relabelled=True))
+
+
 def init_state(g: _graph.CSRGraph, max_delay: int, seed: int, reference_defaults: bool = False):
     rng = np.random.Generator(np.random.PCG64(seed))
     E, N = g.num_edges, g.num_nodes

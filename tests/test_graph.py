@@ -93,3 +93,86 @@ def test_workload_distributions():
     assert 0 <= wl.theta.min() and wl.theta.max() < 2 * np.pi
     assert abs(wl.omega.mean()) < 0.15 and abs(wl.omega.std() - 1) < 0.1
     assert np.array_equal(wl.theta, wl.theta_hat)
+
+
+# ---------------------------------------------------------------------------
+# relabelling and graph files
+# ---------------------------------------------------------------------------
+
+def test_permute_maps_every_edge_and_inverts():
+    from leviathan_v1_amd import graph
+
+    g = graph.barabasi_albert(3000, 4, seed=11)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(g.num_nodes).astype(np.int32)
+    h, emap = graph.permute(g, perm)
+    # new entry (row i, column c) is old entry (row perm[i], column perm[c])
+    rows = np.repeat(np.arange(h.num_nodes), np.diff(h.row_ptr))
+    old_rows = np.repeat(np.arange(g.num_nodes), np.diff(g.row_ptr))
+    assert np.array_equal(old_rows[emap], perm[rows])
+    assert np.array_equal(g.col_idx[emap], perm[h.col_idx])
+    assert np.array_equal(np.sort(emap), np.arange(g.num_edges))
+    # columns ascending within rows
+    d = np.diff(h.col_idx)
+    assert (d[np.diff(rows) == 0] > 0).all()
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size, dtype=np.int32)
+    back, emap2 = graph.permute(h, inv)
+    assert np.array_equal(back.row_ptr, g.row_ptr) and np.array_equal(back.col_idx, g.col_idx)
+    assert np.array_equal(emap[emap2], np.arange(g.num_edges))
+    with pytest.raises(ValueError):
+        graph.permute(g, np.zeros(g.num_nodes, dtype=np.int32))
+
+
+def test_orderings():
+    from leviathan_v1_amd import graph
+
+    g = graph.barabasi_albert(2000, 3, seed=2)
+    o = graph.degree_order(g)
+    assert (np.diff(graph.permute(g, o)[0].degrees()) <= 0).all()
+    # a ring lattice under a random labelling: RCM brings the bandwidth back down to ~k
+    ring = graph.watts_strogatz(4000, 6, 0.0, seed=0)
+    rng = np.random.default_rng(1)
+    shuffled, _ = graph.permute(ring, rng.permutation(4000).astype(np.int32))
+    r, _ = graph.permute(shuffled, graph.rcm_order(shuffled))
+
+    def bandwidth(h):
+        rows = np.repeat(np.arange(h.num_nodes), np.diff(h.row_ptr))
+        return int(np.abs(rows - h.col_idx).max())
+
+    assert bandwidth(shuffled) > 1000 and bandwidth(r) <= 12
+
+
+def test_graph_file_round_trip(tmp_path):
+    from leviathan_v1_amd import graph
+
+    g = graph.watts_strogatz(500, 8, 0.2, seed=3)
+    delays = np.arange(g.num_edges, dtype=np.uint8)
+    graph.save(tmp_path / "g.npz", g, delays=delays)
+    h, extra = graph.load(tmp_path / "g.npz")
+    assert np.array_equal(h.row_ptr, g.row_ptr) and np.array_equal(h.col_idx, g.col_idx)
+    assert np.array_equal(extra["delays"], delays)
+    bad = g.col_idx.copy()
+    bad[5] = 500
+    np.savez(tmp_path / "bad.npz", row_ptr=g.row_ptr, col_idx=bad)
+    with pytest.raises(ValueError):
+        graph.load(tmp_path / "bad.npz")
+
+
+def test_relabelled_workload_steps_like_the_original():
+    """FP64 oracle: one step of a relabelled workload equals the original's step, relabelled
+    (row sums run in a different order, so equality is to rounding)."""
+    import oracle
+    from leviathan_v1_amd import graph, workloads
+
+    wl = workloads.barabasi_albert_workload(1500, 4, 20, seed=5)
+    perm = graph.degree_order(wl.graph)
+    rl = workloads.relabel(wl, perm)
+    a = oracle.COracle(*wl.engine_args(), dtype=np.float64)
+    b = oracle.COracle(*rl.engine_args(), dtype=np.float64)
+    for _ in range(5):
+        a.step(0.05)
+        b.step(0.05)
+    assert np.abs(oracle.circ_err(b.theta, a.theta[perm])).max() < 1e-12
+    _, emap = graph.permute(wl.graph, perm)
+    assert np.abs(b.w - a.w[emap]).max() < 1e-12
