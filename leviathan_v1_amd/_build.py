@@ -3,7 +3,7 @@
 * ``libleviathan_graph.so`` -- host C++ (g++ -O3 -fopenmp): seeded topology
   builders.
 * ``libleviathan_hip.so``   -- the gfx950 HIP kernels and the C-ABI declared in
-  ``include/leviathan.h`` (hipcc --offload-arch=gfx950).
+  ``include/leviathan.h`` and ``include/leviathan_phi.h`` (hipcc --offload-arch=gfx950).
 
 Both land next to this file so they travel with the source tree.
 """
@@ -22,8 +22,9 @@ INCLUDE = os.path.join(ROOT, "include")
 GRAPH_SO = os.path.join(PKG, "libleviathan_graph.so")
 HIP_SO = os.path.join(PKG, "libleviathan_hip.so")
 
-HIP_SOURCES = ["leviathan_api.hip", "leviathan_kernels.hip", "leviathan_binned.hip", "plan.cpp", "reservoir.hip"]
-HIP_HEADERS = ["leviathan_internal.h", "step_kernel.h"]
+HIP_SOURCES = ["leviathan_api.hip", "leviathan_kernels.hip", "leviathan_binned.hip", "plan.cpp", "reservoir.hip",
+               "phi.hip"]
+HIP_HEADERS = ["leviathan_internal.h", "step_kernel.h", "phi_internal.h", "sincos_bounded.h"]
 
 # Packed FP32 instructions are switched off for every device compile.
 HIPCC_FLAGS = [

@@ -1,4 +1,5 @@
-"""ctypes binding of ``libleviathan_hip.so`` (declarations: include/leviathan.h).
+"""ctypes binding of ``libleviathan_hip.so`` (declarations: include/leviathan.h and
+include/leviathan_phi.h).
 
 ``ctypes.CDLL`` releases the GIL for the duration of every foreign call, so
 ``step`` (and every other engine call) runs with the GIL released, as the
@@ -44,7 +45,7 @@ LAYOUT = {"node": 0, "slot": 1}
 SCHEDULE = {"direct": 0, "binned": 1, "auto": 2, "staged": 3}
 PLAN_NAMES = [f"{k}/{e}" for k in ("direct", "binned", "staged") for e in (4, 8, 16)]
 
-# name -> (restype, argtypes); every symbol include/leviathan.h declares.
+# name -> (restype, argtypes); every symbol include/leviathan.h and include/leviathan_phi.h declare.
 SIGNATURES = {
     "leviathan_init": (vp, [c_int, c_int, vp, vp, vp, vp, vp, vp, vp]),
     "leviathan_step": (c_float, [vp, c_float]),
@@ -90,6 +91,19 @@ SIGNATURES = {
     "leviathan_features_colsum": (c_int, [vp, c_int64, c_int64, vp, vp]),
     "leviathan_features_matvec": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
     "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, c_int, vp, vp]),
+    # include/leviathan_phi.h
+    "phi_init": (vp, [c_int, vp]),
+    "phi_accumulate": (None, [vp, vp]),
+    "phi_compute": (c_float, [vp]),
+    "phi_reset": (None, [vp]),
+    "phi_get_transition_count": (c_int64, [vp]),
+    "phi_free": (None, [vp]),
+    "leviathan_attach_phi": (c_int, [vp, vp]),
+    "phi_accumulate_host": (c_int, [vp, vp, c_int64]),
+    "phi_get_counts": (c_int, [vp, vp, vp]),
+    "phi_get_state": (c_int, [vp, vp, vp]),
+    "phi_set_state": (c_int, [vp, vp, vp, c_int, c_int64]),
+    "phi_compute_ex": (c_int, [vp, vp, vp, vp, vp]),
 }
 
 _lib = None

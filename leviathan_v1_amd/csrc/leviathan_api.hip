@@ -16,6 +16,7 @@
 
 #include "leviathan.h"
 #include "leviathan_internal.h"
+#include "leviathan_phi.h"
 #include "plan.h"
 
 using lev::FinalizeArgs;
@@ -140,6 +141,7 @@ struct LeviathanHandle {
   hipStream_t stream = nullptr;
   std::vector<NetCtrl> h_ctrl;  // host copy of the static controller fields
   std::vector<GraphEntry> graphs;
+  PhiHandle* phi = nullptr;     // hub statistics recorded by every step (leviathan_attach_phi)
 };
 
 namespace {
@@ -169,6 +171,7 @@ void destroy(LeviathanHandle* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->phi) lev::phi_link(h->phi, nullptr, nullptr);
   drop_graphs(h);
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_ctrl,    h->d_step,
@@ -251,6 +254,8 @@ FinalizeArgs fin_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   f.n_nets = h->n_nets;
   f.num_blocks = pl.n_items;
   f.dt = dt;
+  f.theta = h->d_theta;
+  if (h->phi) f.phi = lev::phi_view(h->phi);
   return f;
 }
 
@@ -728,6 +733,12 @@ bool refresh_ctrl_field(LeviathanHandle* h, int net, size_t offset, const void* 
 
 }  // namespace
 
+namespace lev {
+void error_set(const char* msg) { g_last_error = msg; }
+void error_clear() { g_last_error.clear(); }
+bool error_empty() { return g_last_error.empty(); }
+}  // namespace lev
+
 extern "C" {
 
 const char* leviathan_last_error(void) { return g_last_error.c_str(); }
@@ -1069,6 +1080,35 @@ int leviathan_set_step_count(LeviathanHandle* h, int64_t step) {
   return 0;
 }
 
+int leviathan_attach_phi(LeviathanHandle* h, PhiHandle* phi) {
+  if (!check_handle(h)) return -1;
+  if (phi == h->phi) return 0;
+  if (phi) {
+    if (lev::phi_device(phi) != h->device) {
+      set_err("attach_phi: the PhiHandle lives on device %d, the engine on %d", lev::phi_device(phi), h->device);
+      return -1;
+    }
+    if (lev::phi_max_hub(phi) >= h->N) {
+      set_err("attach_phi: hub index %d out of range for %d nodes", lev::phi_max_hub(phi), h->N);
+      return -1;
+    }
+    if (lev::phi_engine(phi)) {
+      set_err("attach_phi: the PhiHandle is attached to another engine");
+      return -1;
+    }
+  }
+  const hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_err("attach_phi: %s", hipGetErrorString(e));
+    return -1;
+  }
+  if (h->phi) lev::phi_link(h->phi, nullptr, nullptr);
+  h->phi = phi;
+  if (phi) lev::phi_link(phi, h, h->stream);
+  drop_graphs(h);  // captured finalize arguments name the old handle
+  return 0;
+}
+
 int leviathan_num_nodes(LeviathanHandle* h) { return h ? h->N : 0; }
 int64_t leviathan_num_edges(LeviathanHandle* h) { return h ? h->E : 0; }
 int64_t leviathan_step_count(LeviathanHandle* h) { return h ? h->step : 0; }
@@ -1197,6 +1237,8 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
   }
   const int64_t step0 = h->step;
   const int active0 = h->active;
+  PhiHandle* const phi0 = h->phi;  // trial steps are rolled back: they record no hub transitions
+  h->phi = nullptr;
   double best = 1e300;
   int best_idx = active0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1229,6 +1271,7 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   h->step = step0;
+  h->phi = phi0;
   if (best_idx != active0) drop_graphs(h);
   h->active = ok ? best_idx : active0;
   if (!ok) {

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "phi_internal.h"
+
 namespace lev {
 
 constexpr int kBlock = 256;          // lanes per step workgroup (4 waves of 64)
@@ -78,6 +80,8 @@ struct FinalizeArgs {
   int32_t n_nets;
   int32_t num_blocks;     // partials of the active plan
   float dt;
+  const float* theta;     // phases after the step (read by the hub accumulation)
+  PhiDev phi;             // phi.st null: no hub statistics attached
 };
 
 // Two-phase (binned) step: see leviathan_binned.hip.

@@ -9,7 +9,9 @@
 //                    integrate + history write + FP64 partial sums of
 //                    cos/sin, for one row block per 256-lane workgroup;
 //   finalize_kernel  FP64 sum of the partials of each network, r, the FP64
-//                    gain controller, and the step counter.
+//                    gain controller, the step counter and, when a hub
+//                    statistics handle is attached, the hub state of the
+//                    new phases.
 //
 // The fusion is legal because a row only writes its own state (theta,
 // theta_hat, omega_hat, its incoming edges' weights and its own history
@@ -320,6 +322,12 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
   const int wave = tid >> 6;
   const int64_t step = *a.step;
   const int64_t hslot = step % kRHistSteps;
+  // Hub transition statistics of an attached PhiHandle, from the phases the
+  // row pass just wrote: the last wavefront issues its loads now and records
+  // the state after its share of the reduction.
+  const bool hub_wave = a.phi.st != nullptr && wave == kFinalizeThreads / 64 - 1;
+  HubLoad hub{};
+  if (hub_wave) hub = hub_load(a.phi, a.theta, lane);
 
   auto finish = [&](int net, double c, double s) {
     NetCtrl& nc = a.ctrl[net];
@@ -387,6 +395,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
       if (lane == 0) finish(net, c, s);
     }
   }
+  if (hub_wave) hub_record(a.phi, hub, lane);
   __syncthreads();
   if (tid == 0) *a.step = step + 1;
 }

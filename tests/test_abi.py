@@ -1,4 +1,5 @@
-"""The C-ABI library: it loads, exports every symbol include/leviathan.h declares,
+"""The C-ABI library: it loads, exports every symbol include/leviathan.h and
+include/leviathan_phi.h declare,
 and rejects invalid inputs with a message before touching the GPU (no compute
 call is made here, so these run without one)."""
 import ctypes
@@ -11,14 +12,16 @@ import pytest
 from leviathan_v1_amd import _build, _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "leviathan.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("leviathan.h", "leviathan_phi.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"\b(leviathan_[a-z0-9_]+)\s*\(", text)
-    return sorted(set(names))
+    names = set()
+    for path in HEADERS:
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b((?:leviathan|phi)_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_built():
@@ -42,6 +45,29 @@ def test_reference_entry_points_present():
               "leviathan_set_gain", "leviathan_get_gain", "leviathan_set_gain_controller",
               "leviathan_get_vram_usage", "leviathan_reset_weights"]:
         assert hasattr(lib, n)
+
+
+def test_reference_phi_entry_points_present():
+    """The reference's Phi host API (leviathan_phi_kernel.cu:242-368) under the same names."""
+    lib = ctypes.CDLL(_build.HIP_SO)
+    for n in ["phi_init", "phi_accumulate", "phi_compute", "phi_reset", "phi_get_transition_count", "phi_free"]:
+        assert hasattr(lib, n)
+
+
+@pytest.mark.parametrize("hubs, msg", [([], "num_hubs"), (list(range(11)), "num_hubs"), ([0, -3], "negative")])
+def test_phi_init_rejects_invalid_hubs(hubs, msg):
+    lib = _lib.lib()
+    a = np.asarray(hubs if hubs else [0], dtype=np.int32)
+    assert not lib.phi_init(len(hubs), ctypes.c_void_p(a.ctypes.data))
+    assert msg in _lib.last_error()
+
+
+def test_phi_null_handle_calls_report_error():
+    lib = _lib.lib()
+    assert np.isnan(lib.phi_compute(None))
+    assert "null" in _lib.last_error()
+    assert lib.phi_get_transition_count(None) == -1
+    assert lib.leviathan_attach_phi(None, None) == -1
 
 
 def test_default_config_matches_reference_parameters():
