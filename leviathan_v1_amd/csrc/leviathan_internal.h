@@ -82,7 +82,16 @@ struct FinalizeArgs {
   float dt;
   const float* theta;     // phases after the step (read by the hub accumulation)
   PhiDev phi;             // phi.st null: no hub statistics attached
+  double2* chunks;        // [ceil(num_blocks / kChunkPartials)] chunk sums (one network), or null
 };
+
+// One network with many row blocks: a grid of workgroups first sums chunks of
+// kChunkPartials partials (one round of loads each), and the finalize kernel
+// sums the chunk sums.  A single workgroup reading 10^5 partials is bound by
+// load latency (~30 us on the 10^7-node graph).
+constexpr int kChunkThreads = 256;
+constexpr int kChunkPartials = kChunkThreads * 8;
+constexpr int kChunkMinBlocks = 16384;
 
 // Two-phase (binned) step: see leviathan_binned.hip.
 constexpr int kScatterEpt = 16;     // phase 1: out-edges per lane in a source block

@@ -315,6 +315,41 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 
 // r and the homeostatic gain controller (leviathan_kernel.cu:237-264 and
 // :430-435), in FP64, for every network; then the step counter advances.
+// Sum of kChunkPartials consecutive partials per workgroup, in a fixed order.
+__global__ __launch_bounds__(kChunkThreads) void chunk_sum_kernel(const double2* partials, int32_t n,
+                                                                  double2* chunks) {
+  __shared__ double s_red[2][kChunkThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t base = blockIdx.x * kChunkPartials;
+  double2 p[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int32_t q = base + k * kChunkThreads + tid;
+    p[k] = q < n ? partials[q] : make_double2(0.0, 0.0);
+  }
+  double c = 0.0, s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    c += p[k].x;
+    s += p[k].y;
+  }
+  c = wave_sum(c);
+  s = wave_sum(s);
+  if (lane == 0) {
+    s_red[0][wave] = c;
+    s_red[1][wave] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double tc = 0.0, ts = 0.0;
+    for (int w = 0; w < kChunkThreads / 64; ++w) {
+      tc += s_red[0][w];
+      ts += s_red[1][w];
+    }
+    chunks[blockIdx.x] = make_double2(tc, ts);
+  }
+}
+
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs a) {
   __shared__ double s_red[2][kFinalizeThreads / 64];
   const int tid = threadIdx.x;
@@ -476,7 +511,16 @@ hipError_t launch_step(int history, int layout, int ept, int gather, const StepA
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
-  finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(a);
+  if (a.n_nets == 1 && a.chunks && a.num_blocks >= kChunkMinBlocks) {
+    const int32_t n_chunks = (a.num_blocks + kChunkPartials - 1) / kChunkPartials;
+    chunk_sum_kernel<<<n_chunks, kChunkThreads, 0, s>>>(a.partials, a.num_blocks, a.chunks);
+    FinalizeArgs b = a;
+    b.partials = a.chunks;
+    b.num_blocks = n_chunks;
+    finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(b);
+  } else {
+    finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 

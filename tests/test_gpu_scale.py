@@ -2,7 +2,8 @@
 up to ~10^4), delays uniform in [1, 50], FP32 and uint16 history.
 
   * per-step phases and weights of each form within 1e-5 relative of one FP64 oracle step
-    taken from the engine's exported state (the bar of the other GPU tests);
+    taken from the engine's exported state (the bar of the other GPU tests), and the
+    step's r within 1e-6 of the FP64 order parameter of the phases it wrote;
   * 2,000 steps of the uint16 form (the schedule the autotuner picks) against 2,000 steps of
     the plain FP32 drop-in from the same inputs: windowed (200-step) mean r and r variance
     within 5e-3.  An FP64 run of 4e11 edge-steps is out of reach of the CPU oracle, so
@@ -40,11 +41,16 @@ def test_ba10m_one_step_vs_fp64_oracle(ba10m, history, schedule):
     eng.step_n(DT, 20)
     o = oracle_from_engine(eng, ba10m)
     o.step(DT)
-    eng.step(DT)
-    et, ew = theta_rel_err(eng.get_theta(), o.theta), w_rel_err(eng.get_weights(), o.w)
-    say("theta rel err", et, "w rel err", ew)
+    r = eng.step(DT)
+    th = eng.get_theta()
+    et, ew = theta_rel_err(th, o.theta), w_rel_err(eng.get_weights(), o.w)
+    # r of the step (10^5 row-block partials, summed in chunks, then the chunk sums) against
+    # the FP64 order parameter of the phases the step wrote
+    r_np = float(np.abs(np.exp(1j * th.astype(np.float64)).mean()))
+    say("theta rel err", et, "w rel err", ew, "r", r, "r of phases", r_np, "blocks", eng.num_blocks)
     eng.close()
     assert et < TOL and ew < TOL
+    assert abs(r - r_np) < 1e-6
 
 
 def test_ba10m_u16_windowed_r_matches_fp32(ba10m):
