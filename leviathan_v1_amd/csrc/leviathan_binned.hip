@@ -70,30 +70,39 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
     }
     __syncthreads();
+    // Every load is unconditional (a lane past the block's edges reads out-edge
+    // 0 and the block's first source, and discards the values), so all of a
+    // lane's loads are in flight at once; with a branch around each one the
+    // compiler waited for every history load before issuing the next.
     int32_t dd[EPT];
     int32_t rk[EPT];
     uint32_t pp[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
-        dd[k] = __builtin_nontemporal_load(a.delay + c0 + le);
-        rk[k] = __builtin_nontemporal_load(a.rank + c0 + le);
-        pp[k] = __builtin_nontemporal_load(a.pos + c0 + le);
-      }
+      const int32_t c = le < nnz ? c0 + le : 0;
+      dd[k] = __builtin_nontemporal_load(a.delay + c);
+      rk[k] = __builtin_nontemporal_load(a.rank + c);
+      pp[k] = __builtin_nontemporal_load(a.pos + c);
     }
-    // read by source (lanes of one source share its history line), place in slot order
+    // read by source (lanes of one source share its history line)
+    H v[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
-        int32_t slot = cur - dd[k];
-        slot += (slot < 0) ? S : 0;
-        // node-major: a source's history line is only reused by this block's lanes, so it is
-        // read with the streaming hint and leaves the L2 to the buckets' write fronts
-        const H* src = ring + ring_index<LAYOUT>(pitch, s0 + s_esrc[le], slot);
-        s_val[rk[k]] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
-      }
+      int32_t slot = cur - dd[k];
+      slot += (slot < 0) ? S : 0;
+      const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+      // node-major: a source's history line is only reused by this block's lanes, so it is
+      // read with the streaming hint and leaves the L2 to the buckets' write fronts
+      const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
+      v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+    }
+    // place in slot order
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      if (le < nnz) s_val[rk[k]] = v[k];
     }
     __syncthreads();
     // write in slot order: the values bound for one bucket leave as one contiguous run
@@ -103,13 +112,30 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       if (le < nnz) hbuf[pp[k]] = s_val[le];
     }
   } else {
-    // one source with more than CAP out-edges: the whole workgroup strides over them
-    for (int32_t le = tid; le < nnz; le += kBlock) {
-      const int32_t d = __builtin_nontemporal_load(a.delay + c0 + le);
-      const uint32_t p = __builtin_nontemporal_load(a.pos + c0 + le);
-      int32_t slot = cur - d;
-      slot += (slot < 0) ? S : 0;
-      hbuf[p] = ring[ring_index<LAYOUT>(pitch, s0, slot)];  // a hub's line: reused, cached
+    // one source with more than CAP out-edges: the whole workgroup strides over
+    // them, four edges per lane at a time with unconditional loads
+    for (int32_t base = 0; base < nnz; base += 4 * kBlock) {
+      int32_t d[4];
+      uint32_t p[4];
+      H v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * kBlock + tid;
+        const int32_t c = c0 + (le < nnz ? le : 0);
+        d[k] = __builtin_nontemporal_load(a.delay + c);
+        p[k] = __builtin_nontemporal_load(a.pos + c);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int32_t slot = cur - d[k];
+        slot += (slot < 0) ? S : 0;
+        v[k] = ring[ring_index<LAYOUT>(pitch, s0, slot)];  // a hub's line: reused, cached
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * kBlock + tid;
+        if (le < nnz) hbuf[p[k]] = v[k];
+      }
     }
   }
 }

@@ -125,49 +125,66 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
     }
 
+    // The loads are unconditional: a lane past the block's edges reads edge 0
+    // (every edge array holds at least one element) and discards the value.
+    // Straight-line code keeps all of a lane's loads in flight at once; with a
+    // branch around each load the compiler waited for every load before
+    // issuing the next.
     int32_t jj[EPT];
     int32_t dd[EPT];
     float ww[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
-        if constexpr (GATHER == 0) {
-          jj[k] = a.col[e0 + le];
-          dd[k] = a.delay[e0 + le];
-        } else if constexpr (GATHER == 1) {
-          jj[k] = (int32_t)a.hpos[e0 + le];
-        } else {
-          jj[k] = (int32_t)a.lpos[e0 + le];
-        }
-        ww[k] = a.w[e0 + le];
+      const int32_t e = le < nnz ? e0 + le : 0;
+      if constexpr (GATHER == 0) {
+        jj[k] = a.col[e];
+        dd[k] = a.delay[e];
+      } else if constexpr (GATHER == 1) {
+        jj[k] = (int32_t)a.hpos[e];
+      } else {
+        jj[k] = (int32_t)a.lpos[e];
       }
+      ww[k] = a.w[e];
     }
     if constexpr (GATHER == 2) {
       // s_contrib holds the block's staged values until the edge pass below
+      H sv[EPT];
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * kBlock + tid;
-        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(hbuf[e0 + le]);
+        sv[k] = hbuf[le < nnz ? e0 + le : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(sv[k]);
       }
     }
     __syncthreads();
 
     float hh[EPT];
+    if constexpr (GATHER == 2) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        if (le < nnz) hh[k] = s_contrib[jj[k]];
+      }
+    } else {
+      // every lane's column, delay or slot is a valid one (edge 0 for a lane past the edges)
+      H hr[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
         if constexpr (GATHER == 0) {
           int32_t slot = cur - dd[k];
           slot += (slot < 0) ? S : 0;
-          hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
-        } else if constexpr (GATHER == 1) {
-          hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
+          hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
         } else {
-          hh[k] = s_contrib[jj[k]];
+          hr[k] = hbuf[(uint32_t)jj[k]];
         }
       }
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) hh[k] = HistCodec<H>::decode(hr[k]);
     }
     if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
 #pragma unroll
@@ -228,36 +245,32 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     const float gate = s_gate[0];
     CompSum acc;
     for (int32_t base = 0; base < nnz; base += kBlock * 4) {
+      // unconditional loads (edge e0 for a lane past the row's end), as in stream mode
       int32_t jj[4], dd[4];
-      float ww[4], hh[4];
+      float ww[4];
+      H hr[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int32_t le = base + k * kBlock + tid;
-        if (le < nnz) {
-          if constexpr (GATHER == 0) {
-            jj[k] = a.col[e0 + le];
-            dd[k] = a.delay[e0 + le];
-          } else if constexpr (GATHER == 1) {
-            jj[k] = (int32_t)a.hpos[e0 + le];
-          } else {
-            jj[k] = 0;
-          }
-          ww[k] = a.w[e0 + le];
+        const int32_t e = e0 + (le < nnz ? le : 0);
+        if constexpr (GATHER == 0) {
+          jj[k] = a.col[e];
+          dd[k] = a.delay[e];
+        } else if constexpr (GATHER == 1) {
+          jj[k] = (int32_t)a.hpos[e];
+        } else {
+          jj[k] = e;
         }
+        ww[k] = a.w[e];
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int32_t le = base + k * kBlock + tid;
-        if (le < nnz) {
-          if constexpr (GATHER == 0) {
-            int32_t slot = cur - dd[k];
-            slot += (slot < 0) ? S : 0;
-            hh[k] = HistCodec<H>::decode(ring[ring_index<LAYOUT>(pitch, jj[k], slot)]);
-          } else if constexpr (GATHER == 1) {
-            hh[k] = HistCodec<H>::decode(hbuf[(uint32_t)jj[k]]);
-          } else {
-            hh[k] = HistCodec<H>::decode(hbuf[e0 + le]);
-          }
+        if constexpr (GATHER == 0) {
+          int32_t slot = cur - dd[k];
+          slot += (slot < 0) ? S : 0;
+          hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
+        } else {
+          hr[k] = hbuf[(uint32_t)jj[k]];
         }
       }
 #pragma unroll
@@ -265,7 +278,8 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
           float c;
-          const float wn = edge_update(hh[k], th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+          const float wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min,
+                                       a.w_max, &c);
           acc.add(c);
           a.w[e0 + le] = wn;
         }
@@ -313,8 +327,6 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   }
 }
 
-// r and the homeostatic gain controller (leviathan_kernel.cu:237-264 and
-// :430-435), in FP64, for every network; then the step counter advances.
 // Sum of kChunkPartials consecutive partials per workgroup, in a fixed order.
 __global__ __launch_bounds__(kChunkThreads) void chunk_sum_kernel(const double2* partials, int32_t n,
                                                                   double2* chunks) {
@@ -350,6 +362,8 @@ __global__ __launch_bounds__(kChunkThreads) void chunk_sum_kernel(const double2*
   }
 }
 
+// r and the homeostatic gain controller (leviathan_kernel.cu:237-264 and
+// :430-435), in FP64, for every network; then the step counter advances.
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs a) {
   __shared__ double s_red[2][kFinalizeThreads / 64];
   const int tid = threadIdx.x;
