@@ -76,14 +76,12 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // compiler waited for every history load before issuing the next.
     int32_t dd[EPT];
     int32_t rk[EPT];
-    uint32_t pp[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       const int32_t c = le < nnz ? c0 + le : 0;
       dd[k] = __builtin_nontemporal_load(a.delay + c);
       rk[k] = __builtin_nontemporal_load(a.rank + c);
-      pp[k] = __builtin_nontemporal_load(a.pos + c);
     }
     // read by source (lanes of one source share its history line)
     H v[EPT];
@@ -103,6 +101,13 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) s_val[rk[k]] = v[k];
+    }
+    // the slots, loaded once the delays and ranks are no longer live (fewer registers)
+    uint32_t pp[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * kBlock + tid;
+      pp[k] = __builtin_nontemporal_load(a.pos + (le < nnz ? c0 + le : 0));
     }
     __syncthreads();
     // write in slot order: the values bound for one bucket leave as one contiguous run
