@@ -112,6 +112,7 @@ struct LeviathanHandle {
   // binned schedule arrays
   int32_t* d_csc_ptr = nullptr;
   uint8_t* d_delay_csc = nullptr;
+  uint8_t* d_esrc = nullptr;       // slot-major ring: each CSC edge's source index in its block
   uint32_t* d_pos = nullptr;
   void* d_hbuf = nullptr;    // two-phase schedules: the scatter kernel's bucketed values
   void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block
@@ -176,7 +177,7 @@ void destroy(LeviathanHandle* h) {
   drop_graphs(h);
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_ctrl,    h->d_step,
-                  h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc,
+                  h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_esrc,
                   h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
@@ -265,6 +266,7 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   lev::ScatterArgs sa{};
   sa.csc_ptr = h->d_csc_ptr;
   sa.delay = h->d_delay_csc;
+  sa.esrc = h->d_esrc;
   sa.pos = pl.d_pos;
   sa.rank = pl.d_rank;
   sa.ring = h->d_ring;
@@ -571,6 +573,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       return fail();
     }
     h->n_sources = (int32_t)hsc.sources.size();
+    if (cfg.layout == LEV_LAYOUT_SLOT_MAJOR) lev::order_scatter_by_delay(hsc);
   }
   if (want_binned) {
     if (!lev::bucket_positions(N, row_ptr, col_idx, cfg.bucket_edges, bin_hpos, h->n_buckets, perr) ||
@@ -680,6 +683,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
                 "copy sources") &&
          hip_ok(hipMemcpy(h->d_csc_ptr, hsc.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
          (E == 0 || hip_ok(hipMemcpy(h->d_delay_csc, hsc.delay_csc.data(), E, H2D), "copy csc delays"));
+    if (ok && cfg.layout == LEV_LAYOUT_SLOT_MAJOR)
+      ok = dalloc(h, &h->d_esrc, E) &&
+           (E == 0 || hip_ok(hipMemcpy(h->d_esrc, hsc.src_local.data(), E, H2D), "copy edge sources"));
     if (!ok) return fail();
     if (want_binned) {
       if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E)) return fail();

@@ -53,46 +53,58 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   const int64_t pitch = a.ring_pitch;
 
   if (!(blk.w & kBlockLongRow)) {
-    const int32_t nsrc = (blk.w >> 1) & 511;
-    int32_t rs = 0, re = 0;
-    if (tid < nsrc) {
-      rs = a.csc_ptr[s0 + tid] - c0;
-      re = a.csc_ptr[s0 + tid + 1] - c0;
+    if constexpr (LAYOUT == 0) {
+      // node-major: the block's out-edges are in source order; map each edge to its source
+      const int32_t nsrc = (blk.w >> 1) & 511;
+      int32_t rs = 0, re = 0;
+      if (tid < nsrc) {
+        rs = a.csc_ptr[s0 + tid] - c0;
+        re = a.csc_ptr[s0 + tid + 1] - c0;
+      }
+      // a source with more than kSerialRow out-edges is mapped by its whole wavefront
+      const bool heavy = tid < nsrc && re - rs > kSerialRow;
+      if (tid < nsrc && !heavy)
+        for (int32_t q = rs; q < re; ++q) s_esrc[q] = (uint8_t)tid;
+      const int lane = tid & 63;
+      for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+        const int src = __ffsll((unsigned long long)m) - 1;
+        const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+        for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
+      }
+      __syncthreads();
     }
-    // a source with more than kSerialRow out-edges is mapped by its whole wavefront
-    const bool heavy = tid < nsrc && re - rs > kSerialRow;
-    if (tid < nsrc && !heavy)
-      for (int32_t q = rs; q < re; ++q) s_esrc[q] = (uint8_t)tid;
-    const int lane = tid & 63;
-    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
-      const int src = __ffsll((unsigned long long)m) - 1;
-      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
-      for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
-    }
-    __syncthreads();
     // Every load is unconditional (a lane past the block's edges reads out-edge
     // 0 and the block's first source, and discards the values), so all of a
     // lane's loads are in flight at once; with a branch around each one the
     // compiler waited for every history load before issuing the next.
     int32_t dd[EPT];
     int32_t rk[EPT];
+    int32_t sl[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       const int32_t c = le < nnz ? c0 + le : 0;
       dd[k] = __builtin_nontemporal_load(a.delay + c);
       rk[k] = __builtin_nontemporal_load(a.rank + c);
+      if constexpr (LAYOUT == 1) sl[k] = le < nnz ? (int32_t)__builtin_nontemporal_load(a.esrc + c) : 0;
     }
-    // read by source (lanes of one source share its history line)
+    // node-major: edges in source order, so the lanes of one source share its
+    // history line (read with the streaming hint: only this block reuses it,
+    // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges in
+    // (delay, source) order, so the lanes of one load read one slot row for
+    // consecutive sources.
     H v[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       int32_t slot = cur - dd[k];
       slot += (slot < 0) ? S : 0;
-      const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-      // node-major: a source's history line is only reused by this block's lanes, so it is
-      // read with the streaming hint and leaves the L2 to the buckets' write fronts
+      int32_t node;
+      if constexpr (LAYOUT == 0) {
+        node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+      } else {
+        node = s0 + sl[k];
+      }
       const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
       v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
     }

@@ -115,6 +115,7 @@ struct ScatterArgs {
   const uint8_t* delay;     // [E] delays in CSC order
   const uint32_t* pos;      // [E] value slots: per short source block in slot order, per long source in CSC order
   const uint16_t* rank;     // [E] short source blocks: CSC edge -> its index in the block's slot order
+  const uint8_t* esrc;      // [E] slot-major ring: each edge's source index within its block
   const void* ring;
   int64_t ring_pitch;
   int32_t n_slots;

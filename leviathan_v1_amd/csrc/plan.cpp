@@ -234,6 +234,31 @@ void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std:
   });
 }
 
+void order_scatter_by_delay(HostScatter& sc) {
+  sc.src_local.assign(sc.delay_csc.size(), 0);
+  parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {
+    const int4 it = sc.sources[q];
+    if (it.w & kBlockLongRow) return;
+    const int32_t s0 = it.x, c0 = it.y, c1 = it.z, nsrc = (it.w >> 1) & 511;
+    const int32_t n = c1 - c0;
+    std::vector<uint64_t> key((size_t)n);
+    for (int32_t i = 0; i < nsrc; ++i)
+      for (int32_t c = sc.csc_ptr[s0 + i]; c < sc.csc_ptr[s0 + i + 1]; ++c)
+        key[c - c0] = ((uint64_t)sc.delay_csc[c] << 40) | ((uint64_t)i << 32) | (uint32_t)(c - c0);
+    std::sort(key.begin(), key.end());
+    std::vector<uint8_t> d((size_t)n);
+    std::vector<uint32_t> e((size_t)n);
+    for (int32_t k = 0; k < n; ++k) {
+      const uint32_t l = (uint32_t)(key[k] & 0xffffffffu);
+      d[k] = sc.delay_csc[c0 + l];
+      e[k] = sc.csc_edge[c0 + l];
+      sc.src_local[c0 + k] = (uint8_t)((key[k] >> 32) & 0xff);
+    }
+    std::copy(d.begin(), d.end(), sc.delay_csc.begin() + c0);
+    std::copy(e.begin(), e.end(), sc.csc_edge.begin() + c0);
+  });
+}
+
 bool staged_bucket_positions(int N, const int* row_ptr, const int* col, const HostPlan& plan, int bucket_edges,
                              std::vector<uint32_t>& hpos, std::vector<int32_t>& bucket_begin, std::string& err) {
   const int64_t E = row_ptr[N];

@@ -30,6 +30,7 @@ struct HostScatter {
   std::vector<int32_t> csc_ptr;     // [N+1] out-edge offsets
   std::vector<uint8_t> delay_csc;   // [E] delays in CSC order
   std::vector<uint32_t> csc_edge;   // [E] CSC index -> CSR edge
+  std::vector<uint8_t> src_local;   // [E] after order_scatter_by_delay: each edge's source index in its block
 };
 
 // Row blocks of the direct schedule: runs of <= 256 rows holding <= cap edges,
@@ -89,5 +90,12 @@ bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std
 // block, the slots in ascending order, and rank[c] the index of CSC edge c in
 // that order; long sources keep CSC order (rank unused).
 void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std::vector<uint16_t>& rank);
+
+// For a slot-major history ring: inside each short source block, order the
+// out-edges by (delay, source, destination) -- delay_csc and csc_edge are
+// permuted in place -- so that the lanes of one scatter load read one slot row
+// for consecutive sources, and record each edge's source index in its block.
+// Must run before the positions are derived from csc_edge.
+void order_scatter_by_delay(HostScatter& sc);
 
 }  // namespace lev
