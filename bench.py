@@ -136,7 +136,12 @@ def copy_calibration(nbytes: int, reps: int = 3) -> None:
     log(f"copy calibration: {reps} x {nbytes} bytes device-to-device")
 
 
-def load_traffic(workload, history, schedule):
+def traffic_key(workload, history, layout, schedule):
+    """Key of a workload's record in profiles/traffic.json."""
+    return f"{workload}:{history}:{schedule}" if layout == "node" else f"{workload}:{history}:{layout}:{schedule}"
+
+
+def load_traffic(workload, history, layout, schedule):
     """HBM bytes per step of the step's kernels, from the rocprofv3 counter runs
     summarised in profiles/traffic.json (None if that workload was not profiled)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -144,7 +149,7 @@ def load_traffic(workload, history, schedule):
         return None
     try:
         with open(path) as f:
-            rec = json.load(f).get(f"{workload}:{history}:{schedule}")
+            rec = json.load(f).get(traffic_key(workload, history, layout, schedule))
     except (OSError, ValueError):
         return None
     if not rec:
@@ -161,7 +166,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble", "reservoir"])
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
-    ap.add_argument("--layout", default="node", choices=["node", "slot"])
+    ap.add_argument("--layout", default="slot", choices=["node", "slot"],
+                    help="history ring of the f32-tuned and u16 forms (the plain drop-in is node-major, as "
+                         "leviathan_init configures it)")
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned", "staged"])
     ap.add_argument("--bucket-edges", type=int, default=0)
@@ -231,7 +238,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded graph and initial state)",
         "config": {"workload": wl.name, "N": wl.N, "E": wl.E, "max_delay": wl.max_delay,
-                   "history": hist, "layout": args.layout, "schedule": head["schedule"]},
+                   "history": hist, "layout": head["layout"], "schedule": head["schedule"]},
         "hbm_gbs": head["hbm_gbs"],
         "plain_fp32": forms.get("f32"),
         "fp32_tuned": forms.get("f32-tuned"),
@@ -242,7 +249,7 @@ def main():
                                       if "u16" in forms and "f32-tuned" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic(args.workload, hist, head["schedule"]),
+                     "traffic": load_traffic(args.workload, hist, head["layout"], head["schedule"]),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }
