@@ -167,8 +167,8 @@ def main():
     ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble", "reservoir"])
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
     ap.add_argument("--layout", default="slot", choices=["node", "slot"],
-                    help="history ring of the f32-tuned and u16 forms (the plain drop-in is node-major, as "
-                         "leviathan_init configures it)")
+                    help="history ring of the f32-tuned and u16 forms (the plain drop-in uses leviathan_init's, "
+                         "slot-major)")
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned", "staged"])
     ap.add_argument("--bucket-edges", type=int, default=0)
@@ -207,7 +207,8 @@ def main():
     forms = {}
     for f in args.forms.split(","):
         if f == "f32":
-            forms[f] = run_form(wl, "f32", args.steps, args.warmup, "node", 8, "direct")
+            # the plain drop-in exactly as leviathan_init configures it
+            forms[f] = run_form(wl, "f32", args.steps, args.warmup, "slot", 8, "direct")
         else:
             hist = "f32" if f == "f32-tuned" else f
             forms[f] = run_form(wl, hist, args.steps, args.warmup, args.layout, args.ept, args.schedule,

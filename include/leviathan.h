@@ -42,7 +42,8 @@ typedef struct LeviathanHandle LeviathanHandle;
  * (row_ptr[N+1], col_idx[E], delays[E], E = row_ptr[N]) and the initial state
  * (weights[E], theta[N], theta_hat[N], omega[N]; omega_hat starts as omega)
  * to the GPU, with the parameter defaults of :281-297, FP32 history in the
- * node-major layout.  Returns NULL on failure. */
+ * slot-major layout (the faster one for every schedule measured; the layout
+ * is internal and does not change any result).  Returns NULL on failure. */
 LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr,
                                 const int* h_col_idx, const uint8_t* h_delays,
                                 const float* h_weights, const float* h_theta,

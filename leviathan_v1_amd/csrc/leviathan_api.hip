@@ -771,7 +771,7 @@ void leviathan_config_default(LeviathanConfig* c) {
   c->g_max = 5.0f;
   c->gain_controller = 1;
   c->history = LEV_HISTORY_F32;
-  c->layout = LEV_LAYOUT_NODE_MAJOR;
+  c->layout = LEV_LAYOUT_SLOT_MAJOR;
   c->edges_per_thread = 8;
   c->graph_chunk = 100;
   c->schedule = LEV_SCHEDULE_DIRECT;

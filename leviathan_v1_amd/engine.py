@@ -46,7 +46,7 @@ class LeviathanEngine:
 
     Keyword extensions:
       history: "f32" (default) or "u16" (exact-wrap uint16 angles)
-      layout:  "node" (ring[node][slot], default) or "slot" (ring[slot][node])
+      layout:  "slot" (ring[slot][node], default) or "node" (ring[node][slot])
       edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
       schedule: "direct" (one fused kernel, per-edge gather), "binned" (source-
                ordered scatter into destination buckets, then the fused row
@@ -60,7 +60,7 @@ class LeviathanEngine:
     """
 
     def __init__(self, N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega, *,
-                 history: str = "f32", layout: str = "node", edges_per_thread: int = 8, graph_chunk: int = 100,
+                 history: str = "f32", layout: str = "slot", edges_per_thread: int = 8, graph_chunk: int = 100,
                  schedule: str = "direct", bucket_edges: int = 0,
                  params: Optional[dict] = None, n_nets: int = 1, net_offsets: Optional[Sequence[int]] = None,
                  g0: Optional[Sequence[float]] = None, controller: Optional[Sequence[bool]] = None):
@@ -79,7 +79,7 @@ class LeviathanEngine:
         omega = _arr(omega, np.float32, N, "omega")
         self.N, self.E, self.max_delay = N, E, int(max_delay)
         self.n_nets = int(n_nets)
-        extended = (history != "f32" or layout != "node" or edges_per_thread != 8 or graph_chunk != 100
+        extended = (history != "f32" or layout != "slot" or edges_per_thread != 8 or graph_chunk != 100
                     or schedule != "direct" or bucket_edges
                     or params or n_nets != 1 or net_offsets is not None or g0 is not None
                     or controller is not None)

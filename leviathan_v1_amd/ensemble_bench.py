@@ -22,7 +22,8 @@ def _log(*a):
     print("[ensemble]", *a, file=sys.stderr, flush=True)
 
 
-def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 10, history: str = "u16"):
+def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 10, history: str = "u16",
+        layout: str = "slot"):
     t0 = time.perf_counter()
     spec = sweep_spec(count)
     wls = build_workloads(spec)
@@ -30,7 +31,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
     E = sum(w.E for w in wls)
     _log(f"{count} networks, N={N} E={E}, built in {time.perf_counter() - t0:.1f} s")
 
-    ens = LeviathanEnsemble(wls, spec.gains, history=history, schedule="auto", graph_chunk=100)
+    ens = LeviathanEnsemble(wls, spec.gains, history=history, layout=layout, schedule="auto", graph_chunk=100)
     tuned = ens.engine.autotune(ens.dt, steps=5)
     chosen = ens.engine.schedule_name
     _log("autotune", tuned, "->", chosen)
@@ -74,7 +75,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
         "dtype": "f32",
         "data": "synthetic (seeded graphs and initial states)",
         "config": {"workload": f"{count} Watts-Strogatz networks k=20 p=0.1, sizes log-uniform [2000, 200000], "
-                               f"K=linspace(0,4), controller off", "N": N, "E": E, "history": history,
+                               f"K=linspace(0,4), controller off", "N": N, "E": E, "history": history, "layout": layout,
                    "schedule": chosen, "graph_chunk": 100},
         "autotune_ms_per_step": tuned,
         "step_kernel_ms": prof["step_kernel_ms"] + prof["scatter_kernel_ms"] + prof["regroup_kernel_ms"],
