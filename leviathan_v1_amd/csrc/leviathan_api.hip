@@ -107,6 +107,7 @@ struct LeviathanHandle {
   double2* d_partials = nullptr;
   int32_t n_partials = 0;
   double2* d_chunks = nullptr;  // chunk sums of the partials (finalize of one network)
+  unsigned int* d_fin_done = nullptr;  // finalize of several networks: workgroups done this step
   DevPlan plans[kNumPlans];
   int active = kPlanDirect8;
   // binned schedule arrays
@@ -176,7 +177,7 @@ void destroy(LeviathanHandle* h) {
   if (h->phi) lev::phi_link(h->phi, nullptr, nullptr);
   drop_graphs(h);
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
-                  h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_ctrl,    h->d_step,
+                  h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_fin_done, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_esrc,
                   h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
@@ -259,6 +260,7 @@ FinalizeArgs fin_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   f.theta = h->d_theta;
   if (h->phi) f.phi = lev::phi_view(h->phi);
   f.chunks = h->d_chunks;
+  f.fin_done = h->d_fin_done;
   return f;
 }
 
@@ -639,6 +641,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       !dalloc(h, &h->d_omega, (size_t)N) || !dalloc(h, &h->d_omega_hat, (size_t)N) ||
       !dalloc(h, (char**)&h->d_ring, ring_elems * hsize) || !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
       !dalloc(h, &h->d_chunks, (size_t)(h->n_partials + lev::kChunkPartials - 1) / lev::kChunkPartials) ||
+      !dalloc(h, &h->d_fin_done, 0) ||
       !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
       !dalloc(h, &h->d_step, 1) || !dalloc(h, &h->d_r_hist, (size_t)lev::kRHistSteps * n_nets) ||
       !dalloc(h, &h->d_scratch, (size_t)N) || !dalloc(h, &h->d_drive_u, (size_t)lev::kDriveSteps))

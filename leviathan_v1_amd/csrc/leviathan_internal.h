@@ -83,6 +83,7 @@ struct FinalizeArgs {
   const float* theta;     // phases after the step (read by the hub accumulation)
   PhiDev phi;             // phi.st null: no hub statistics attached
   double2* chunks;        // [ceil(num_blocks / kChunkPartials)] chunk sums (one network), or null
+  unsigned int* fin_done; // several networks: workgroups of the finalize grid that have finished (0 between steps)
 };
 
 // One network with many row blocks: a grid of workgroups first sums chunks of

@@ -372,9 +372,9 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
   const int64_t step = *a.step;
   const int64_t hslot = step % kRHistSteps;
   // Hub transition statistics of an attached PhiHandle, from the phases the
-  // row pass just wrote: the last wavefront issues its loads now and records
-  // the state after its share of the reduction.
-  const bool hub_wave = a.phi.st != nullptr && wave == kFinalizeThreads / 64 - 1;
+  // row pass just wrote: the last wavefront (of workgroup 0) issues its loads
+  // now and records the state after its share of the reduction.
+  const bool hub_wave = a.phi.st != nullptr && blockIdx.x == 0 && wave == kFinalizeThreads / 64 - 1;
   HubLoad hub{};
   if (hub_wave) hub = hub_load(a.phi, a.theta, lane);
 
@@ -426,19 +426,33 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
       finish(0, tc, ts);
     }
   } else {
-    for (int net = wave; net < a.n_nets; net += kFinalizeThreads / 64) {
+    // one wavefront per network, spread over a grid of workgroups; each lane
+    // adds the partials q = lane, lane + 64, ... of the network's long and
+    // stream ranges in index order, eight loads in flight
+    const int net = blockIdx.x * (kFinalizeThreads / 64) + wave;
+    if (net < a.n_nets) {
       const int4 rg = a.ranges[net];
       double c = 0.0, s = 0.0;
-      for (int32_t q = rg.x + lane; q < rg.y; q += 64) {
-        const double2 p = a.partials[q];
-        c += p.x;
-        s += p.y;
-      }
-      for (int32_t q = rg.z + lane; q < rg.w; q += 64) {
-        const double2 p = a.partials[q];
-        c += p.x;
-        s += p.y;
-      }
+      auto add_range = [&](int32_t lo, int32_t hi) {
+        int32_t q = lo + lane;
+        for (; q + 7 * 64 < hi; q += 8 * 64) {
+          double2 p[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) p[k] = a.partials[q + k * 64];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            c += p[k].x;
+            s += p[k].y;
+          }
+        }
+        for (; q < hi; q += 64) {
+          const double2 p = a.partials[q];
+          c += p.x;
+          s += p.y;
+        }
+      };
+      add_range(rg.x, rg.y);
+      add_range(rg.z, rg.w);
       c = wave_sum(c);
       s = wave_sum(s);
       if (lane == 0) finish(net, c, s);
@@ -446,7 +460,19 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(FinalizeArgs
   }
   if (hub_wave) hub_record(a.phi, hub, lane);
   __syncthreads();
-  if (tid == 0) *a.step = step + 1;
+  if (tid == 0) {
+    if (gridDim.x == 1) {
+      *a.step = step + 1;
+    } else {
+      // every workgroup has read the step counter (at its start, before the
+      // barrier above); the last one to arrive advances it and re-arms the count
+      const unsigned int arrived = atomicAdd(a.fin_done, 1u);
+      if (arrived == gridDim.x - 1) {
+        *a.fin_done = 0u;
+        *a.step = step + 1;
+      }
+    }
+  }
 }
 
 template <class H, int LAYOUT>
@@ -532,6 +558,9 @@ hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
     b.partials = a.chunks;
     b.num_blocks = n_chunks;
     finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(b);
+  } else if (a.n_nets > 1) {
+    constexpr int kNetsPerWg = kFinalizeThreads / 64;
+    finalize_kernel<<<(a.n_nets + kNetsPerWg - 1) / kNetsPerWg, kFinalizeThreads, 0, s>>>(a);
   } else {
     finalize_kernel<<<1, kFinalizeThreads, 0, s>>>(a);
   }
