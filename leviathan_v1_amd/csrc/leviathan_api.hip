@@ -70,6 +70,9 @@ struct DevPlan {
   bool owns = true;            // binned and staged plans share the row blocks of the direct plan with the same ept
   uint32_t* d_pos = nullptr;   // two-phase plans: [E] value slots in the scatter kernel's write order
   uint16_t* d_rank = nullptr;  // two-phase plans: [E] CSC edge -> index in its source block's write order
+  uint16_t* d_run_idx = nullptr;  // two-phase plans: the scatter's runs of consecutive slots
+  int32_t* d_run_ptr = nullptr;
+  int32_t* d_run_base = nullptr;
   uint16_t* d_lpos = nullptr;  // staged plans: [E] CSR edge -> slot within its block's range
   int4* d_rg_chunks = nullptr; // staged plans: regroup chunks, ranks and runs
   uint16_t* d_rg_rank = nullptr;
@@ -113,12 +116,15 @@ struct LeviathanHandle {
   // binned schedule arrays
   int32_t* d_csc_ptr = nullptr;
   uint8_t* d_delay_csc = nullptr;
-  uint8_t* d_esrc = nullptr;       // slot-major ring: each CSC edge's source index in its block
+  uint16_t* d_dsrc = nullptr;      // slot-major ring: each CSC edge's source index in its block << 8 | delay
   uint32_t* d_pos = nullptr;
   void* d_hbuf = nullptr;    // two-phase schedules: the scatter kernel's bucketed values
   void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block
   uint32_t* d_hpos = nullptr;
   uint16_t* d_rank = nullptr;
+  uint16_t* d_run_idx = nullptr;  // binned plans' scatter runs
+  int32_t* d_run_ptr = nullptr;
+  int32_t* d_run_base = nullptr;
   int4* d_sources = nullptr;  // phase-1 work items
   int32_t n_sources = 0;
   int32_t n_buckets = 0;
@@ -165,6 +171,23 @@ bool dalloc(LeviathanHandle* h, T** p, size_t count) {
   return true;
 }
 
+bool upload_runs(LeviathanHandle* h, const lev::HostScatterRuns& r, uint16_t** idx, int32_t** ptr, int32_t** base) {
+  if (!dalloc(h, idx, r.run_idx.size()) || !dalloc(h, ptr, r.run_ptr.size()) || !dalloc(h, base, r.run_base.size()))
+    return false;
+  hipError_t e = hipSuccess;
+  if (!r.run_idx.empty())
+    e = hipMemcpy(*idx, r.run_idx.data(), sizeof(uint16_t) * r.run_idx.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(*ptr, r.run_ptr.data(), sizeof(int32_t) * r.run_ptr.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(*base, r.run_base.data(), sizeof(int32_t) * r.run_base.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err("copy scatter runs: %s", hipGetErrorString(e));
+    return false;
+  }
+  return true;
+}
+
 void drop_graphs(LeviathanHandle* h) {
   for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
   h->graphs.clear();
@@ -178,14 +201,15 @@ void destroy(LeviathanHandle* h) {
   drop_graphs(h);
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_fin_done, h->d_ctrl,    h->d_step,
-                  h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_esrc,
-                  h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_sources,  h->d_feat_nodes,
+                  h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_dsrc,
+                  h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_run_idx, h->d_run_ptr, h->d_run_base, h->d_sources,  h->d_feat_nodes,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
     if (pl.owns_pos) {
-      void* pp[] = {pl.d_pos, pl.d_rank, pl.d_lpos, pl.d_rg_chunks, pl.d_rg_rank, pl.d_rg_runs};
+      void* pp[] = {pl.d_pos,   pl.d_rank,      pl.d_run_idx,  pl.d_run_ptr, pl.d_run_base,
+                    pl.d_lpos,  pl.d_rg_chunks, pl.d_rg_rank,  pl.d_rg_runs};
       for (void* p : pp)
         if (p) (void)hipFree(p);
     }
@@ -268,9 +292,12 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   lev::ScatterArgs sa{};
   sa.csc_ptr = h->d_csc_ptr;
   sa.delay = h->d_delay_csc;
-  sa.esrc = h->d_esrc;
+  sa.dsrc = h->d_dsrc;
   sa.pos = pl.d_pos;
   sa.rank = pl.d_rank;
+  sa.run_idx = pl.d_run_idx;
+  sa.run_ptr = pl.d_run_ptr;
+  sa.run_base = pl.d_run_base;
   sa.ring = h->d_ring;
   sa.ring_pitch = h->ring_pitch;
   sa.n_slots = h->n_slots;
@@ -566,6 +593,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   lev::HostScatter hsc;
   std::vector<uint32_t> bin_pos, bin_hpos, st_pos[3];
   std::vector<uint16_t> bin_rank, st_rank[3], st_lpos[3];
+  lev::HostScatterRuns bin_runs, st_runs[3];
   lev::HostRegroup st_rg[3];
   const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
   const bool want_staged = cfg.schedule == LEV_SCHEDULE_STAGED || cfg.schedule == LEV_SCHEDULE_AUTO;
@@ -584,6 +612,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       return fail();
     }
     lev::sort_scatter_writes(hsc, bin_pos, bin_rank);
+    lev::encode_scatter_runs(hsc, bin_pos, bin_runs);
     for (int q = 0; q < 3; ++q) {
       DevPlan& pb = h->plans[kPlanBinned + q];
       pb.built = true;
@@ -607,6 +636,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         return fail();
       }
       lev::sort_scatter_writes(hsc, st_pos[q], st_rank[q]);
+      lev::encode_scatter_runs(hsc, st_pos[q], st_runs[q]);
       DevPlan& ps = h->plans[kPlanStaged + q];
       ps.built = true;
       ps.kind = LEV_SCHEDULE_STAGED;
@@ -687,11 +717,13 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
          hip_ok(hipMemcpy(h->d_csc_ptr, hsc.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
          (E == 0 || hip_ok(hipMemcpy(h->d_delay_csc, hsc.delay_csc.data(), E, H2D), "copy csc delays"));
     if (ok && cfg.layout == LEV_LAYOUT_SLOT_MAJOR)
-      ok = dalloc(h, &h->d_esrc, E) &&
-           (E == 0 || hip_ok(hipMemcpy(h->d_esrc, hsc.src_local.data(), E, H2D), "copy edge sources"));
+      ok = dalloc(h, &h->d_dsrc, E) &&
+           (E == 0 || hip_ok(hipMemcpy(h->d_dsrc, hsc.dsrc.data(), sizeof(uint16_t) * E, H2D), "copy edge sources"));
     if (!ok) return fail();
     if (want_binned) {
-      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E)) return fail();
+      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E) ||
+          !upload_runs(h, bin_runs, &h->d_run_idx, &h->d_run_ptr, &h->d_run_base))
+        return fail();
       if (E && !(hip_ok(hipMemcpy(h->d_pos, bin_pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
                  hip_ok(hipMemcpy(h->d_hpos, bin_hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots") &&
                  hip_ok(hipMemcpy(h->d_rank, bin_rank.data(), sizeof(uint16_t) * E, H2D), "copy ranks")))
@@ -702,6 +734,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         pb.d_ranges = h->plans[q].d_ranges;
         pb.d_pos = h->d_pos;
         pb.d_rank = h->d_rank;
+        pb.d_run_idx = h->d_run_idx;
+        pb.d_run_ptr = h->d_run_ptr;
+        pb.d_run_base = h->d_run_base;
       }
     }
     if (want_staged) {
@@ -713,6 +748,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         const lev::HostRegroup& rg = st_rg[q];
         ps.n_rg_chunks = (int32_t)rg.chunks.size();
         if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E) ||
+            !upload_runs(h, st_runs[q], &ps.d_run_idx, &ps.d_run_ptr, &ps.d_run_base) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
             !dalloc(h, &ps.d_rg_runs, rg.runs.size()))
           return fail();

@@ -40,7 +40,8 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   constexpr int EPT = kScatterEpt;
   constexpr int CAP = kBlock * EPT;
   __shared__ uint8_t s_esrc[CAP];
-  __shared__ H s_val[CAP];  // the block's values in slot order
+  __shared__ H s_val[CAP];                  // the block's values in slot order
+  __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
   const int4 blk = a.blocks[b];  // carries its out-edge range: no csc_ptr round trip first
@@ -77,16 +78,28 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // 0 and the block's first source, and discards the values), so all of a
     // lane's loads are in flight at once; with a branch around each one the
     // compiler waited for every history load before issuing the next.
+    // the block's runs of consecutive slots: slot-order index k of run r goes
+    // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
+    const int32_t rb = a.run_ptr[b], nr = min(a.run_ptr[b + 1] - rb, kScatterRuns);
+    int32_t tb[kScatterRuns / kBlock];
+#pragma unroll
+    for (int j = 0; j < kScatterRuns / kBlock; ++j) {
+      const int32_t r = j * kBlock + tid;
+      tb[j] = a.run_base[r < nr ? rb + r : 0];
+    }
+    // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
     int32_t dd[EPT];
     int32_t rk[EPT];
-    int32_t sl[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
       const int32_t c = le < nnz ? c0 + le : 0;
-      dd[k] = __builtin_nontemporal_load(a.delay + c);
+      if constexpr (LAYOUT == 0) {
+        dd[k] = __builtin_nontemporal_load(a.delay + c);
+      } else {
+        dd[k] = le < nnz ? (int32_t)__builtin_nontemporal_load(a.dsrc + c) : 0;
+      }
       rk[k] = __builtin_nontemporal_load(a.rank + c);
-      if constexpr (LAYOUT == 1) sl[k] = le < nnz ? (int32_t)__builtin_nontemporal_load(a.esrc + c) : 0;
     }
     // node-major: edges in source order, so the lanes of one source share its
     // history line (read with the streaming hint: only this block reuses it,
@@ -97,13 +110,13 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      int32_t slot = cur - dd[k];
+      int32_t slot = cur - (dd[k] & 255);
       slot += (slot < 0) ? S : 0;
       int32_t node;
       if constexpr (LAYOUT == 0) {
         node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
       } else {
-        node = s0 + sl[k];
+        node = s0 + (dd[k] >> 8);
       }
       const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
       v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
@@ -114,19 +127,28 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       const int32_t le = k * kBlock + tid;
       if (le < nnz) s_val[rk[k]] = v[k];
     }
-    // the slots, loaded once the delays and ranks are no longer live (fewer registers)
-    uint32_t pp[EPT];
+#pragma unroll
+    for (int j = 0; j < kScatterRuns / kBlock; ++j) {
+      const int32_t r = j * kBlock + tid;
+      if (r < nr) s_base[r] = tb[j];
+    }
+    // each slot-order index's run, loaded once the delays and ranks are no
+    // longer live (fewer registers)
+    int32_t ri[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      pp[k] = __builtin_nontemporal_load(a.pos + (le < nnz ? c0 + le : 0));
+      ri[k] = __builtin_nontemporal_load(a.run_idx + (le < nnz ? c0 + le : 0));
     }
     __syncthreads();
     // write in slot order: the values bound for one bucket leave as one contiguous run
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
-      if (le < nnz) hbuf[pp[k]] = s_val[le];
+      if (le < nnz) {
+        const int32_t base = ri[k] < kScatterRuns ? s_base[ri[k]] : a.run_base[rb + ri[k]];
+        hbuf[base + le] = s_val[le];
+      }
     }
   } else {
     // one source with more than CAP out-edges: the whole workgroup strides over

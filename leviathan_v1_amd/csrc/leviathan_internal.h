@@ -111,12 +111,17 @@ struct RegroupArgs {
 };
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
 
+constexpr int kScatterRuns = 1024;  // runs of a short source block whose slot bases are staged in LDS
+static_assert(kScatterRuns % kBlock == 0, "run bases are staged kBlock at a time");
 struct ScatterArgs {
   const int32_t* csc_ptr;   // [N+1] out-edge offsets per source
   const uint8_t* delay;     // [E] delays in CSC order
-  const uint32_t* pos;      // [E] value slots: per short source block in slot order, per long source in CSC order
+  const uint32_t* pos;      // [E] long sources: value slots in CSC order
   const uint16_t* rank;     // [E] short source blocks: CSC edge -> its index in the block's slot order
-  const uint8_t* esrc;      // [E] slot-major ring: each edge's source index within its block
+  const uint16_t* run_idx;  // [E] short source blocks: each slot-order index's run within its block
+  const int32_t* run_ptr;   // [num_blocks + 1] each block's runs of consecutive slots
+  const int32_t* run_base;  // slot-order index k of run r goes to slot run_base[r] + k
+  const uint16_t* dsrc;     // [E] slot-major ring: each edge's source index in its block << 8 | delay
   const void* ring;
   int64_t ring_pitch;
   int32_t n_slots;

@@ -234,8 +234,35 @@ void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std:
   });
 }
 
+void encode_scatter_runs(const HostScatter& sc, const std::vector<uint32_t>& pos, HostScatterRuns& out) {
+  const int64_t n_items = (int64_t)sc.sources.size();
+  out.run_idx.assign(pos.size(), 0);
+  out.run_ptr.assign((size_t)n_items + 1, 0);
+  auto is_head = [&](int32_t c, int32_t c0) { return c == c0 || pos[c] != pos[c - 1] + 1; };
+  parallel_for(n_items, [&](int64_t q) {
+    if (sc.sources[q].w & kBlockLongRow) return;
+    int32_t runs = 0;
+    for (int32_t c = sc.sources[q].y; c < sc.sources[q].z; ++c) runs += is_head(c, sc.sources[q].y) ? 1 : 0;
+    out.run_ptr[q + 1] = runs;
+  });
+  for (int64_t q = 0; q < n_items; ++q) out.run_ptr[q + 1] += out.run_ptr[q];
+  out.run_base.assign((size_t)std::max(out.run_ptr[n_items], 1), 0);
+  parallel_for(n_items, [&](int64_t q) {
+    if (sc.sources[q].w & kBlockLongRow) return;
+    const int32_t c0 = sc.sources[q].y, c1 = sc.sources[q].z;
+    int32_t r = -1;
+    for (int32_t c = c0; c < c1; ++c) {
+      if (is_head(c, c0)) {
+        ++r;
+        out.run_base[out.run_ptr[q] + r] = (int32_t)((int64_t)pos[c] - (c - c0));
+      }
+      out.run_idx[c] = (uint16_t)r;  // r < 4096: a short block holds at most 4096 edges
+    }
+  });
+}
+
 void order_scatter_by_delay(HostScatter& sc) {
-  sc.src_local.assign(sc.delay_csc.size(), 0);
+  sc.dsrc.assign(sc.delay_csc.size(), 0);
   parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {
     const int4 it = sc.sources[q];
     if (it.w & kBlockLongRow) return;
@@ -252,7 +279,7 @@ void order_scatter_by_delay(HostScatter& sc) {
       const uint32_t l = (uint32_t)(key[k] & 0xffffffffu);
       d[k] = sc.delay_csc[c0 + l];
       e[k] = sc.csc_edge[c0 + l];
-      sc.src_local[c0 + k] = (uint8_t)((key[k] >> 32) & 0xff);
+      sc.dsrc[c0 + k] = (uint16_t)((((key[k] >> 32) & 0xff) << 8) | d[k]);
     }
     std::copy(d.begin(), d.end(), sc.delay_csc.begin() + c0);
     std::copy(e.begin(), e.end(), sc.csc_edge.begin() + c0);

@@ -30,7 +30,7 @@ struct HostScatter {
   std::vector<int32_t> csc_ptr;     // [N+1] out-edge offsets
   std::vector<uint8_t> delay_csc;   // [E] delays in CSC order
   std::vector<uint32_t> csc_edge;   // [E] CSC index -> CSR edge
-  std::vector<uint8_t> src_local;   // [E] after order_scatter_by_delay: each edge's source index in its block
+  std::vector<uint16_t> dsrc;       // [E] after order_scatter_by_delay: source index in the block << 8 | delay
 };
 
 // Row blocks of the direct schedule: runs of <= 256 rows holding <= cap edges,
@@ -91,10 +91,22 @@ bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std
 // that order; long sources keep CSC order (rank unused).
 void sort_scatter_writes(const HostScatter& sc, std::vector<uint32_t>& pos, std::vector<uint16_t>& rank);
 
+// Runs of consecutive slots in each short source block's slot order (after
+// sort_scatter_writes): slot-order index k belongs to run run_idx[c0 + k] of
+// its block and goes to slot run_base[run_ptr[block] + run_idx] + k, so the
+// scatter kernel reads 2 bytes per value instead of a 32-bit slot.
+struct HostScatterRuns {
+  std::vector<uint16_t> run_idx;  // [E]
+  std::vector<int32_t> run_ptr;   // [source items + 1]
+  std::vector<int32_t> run_base;  // [runs]
+};
+void encode_scatter_runs(const HostScatter& sc, const std::vector<uint32_t>& pos, HostScatterRuns& out);
+
 // For a slot-major history ring: inside each short source block, order the
 // out-edges by (delay, source, destination) -- delay_csc and csc_edge are
 // permuted in place -- so that the lanes of one scatter load read one slot row
-// for consecutive sources, and record each edge's source index in its block.
+// for consecutive sources, and record each edge's source index in its block
+// together with its delay.
 // Must run before the positions are derived from csc_edge.
 void order_scatter_by_delay(HostScatter& sc);
 
