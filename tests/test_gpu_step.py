@@ -471,6 +471,36 @@ def test_binned_equals_direct(ba200k, history, layout, schedule):
     assert np.abs(b.get_weights() - d.get_weights()).max() < 1e-6
 
 
+@pytest.fixture(scope="module")
+def ws3k_d0():
+    """WS N=3,000 with max_delay 5 and delays uniform in [0, 5]: delay 0 reads the phases the
+    previous step wrote, delay 5 the oldest slot; the rows hold 20 edges, each source's edges
+    span all six delays."""
+    import dataclasses
+
+    from leviathan_v1_amd import workloads
+
+    wl = workloads.watts_strogatz_workload(3000, 20, 0.1, 5, seed=21)
+    rng = np.random.default_rng(5)
+    return dataclasses.replace(wl, delays=rng.integers(0, 6, wl.E).astype(np.uint8))
+
+
+@pytest.mark.parametrize("schedule", ["direct", "binned", "staged"])
+@pytest.mark.parametrize("layout", ["node", "slot"])
+@pytest.mark.parametrize("history", ["f32", "u16"])
+def test_delay_zero_every_schedule_vs_fp64_oracle(ws3k_d0, schedule, layout, history):
+    """Every schedule and ring layout with delays 0..D against one FP64 oracle step taken from
+    the engine's state after 12 steps (more than two trips round the 7-slot ring)."""
+    say("delay 0..5", schedule, layout, history)
+    eng = make(ws3k_d0, schedule=schedule, layout=layout, history=history)
+    eng.step_n(DT, 12)
+    o = oracle_from_engine(eng, ws3k_d0)
+    o.step(DT)
+    eng.step(DT)
+    assert theta_rel_err(eng.get_theta(), o.theta) < TOL
+    assert w_rel_err(eng.get_weights(), o.w) < TOL
+
+
 @pytest.mark.parametrize("schedule,bucket_edges,ept", [("binned", 0, 8), ("binned", 2048, 8), ("staged", 0, 4),
                                                        ("staged", 0, 16)])
 def test_binned_one_step_consistency(ws1k, schedule, bucket_edges, ept):
