@@ -162,7 +162,8 @@ def load_traffic(workload, history, layout, schedule):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 200; the reservoir workload collects 50,000 driven steps)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble", "reservoir"])
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
@@ -180,6 +181,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (profiler counter scaling)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50_000 if args.workload == "reservoir" else 200
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU "
               "path: the step shards by node ranges, which is described but not run.", file=sys.stderr)

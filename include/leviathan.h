@@ -252,6 +252,20 @@ int leviathan_features_matvec(LeviathanHandle* handle, int64_t row0, int64_t nro
 int leviathan_features_gram(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean, int terms,
                             float* gram_out, double* ms_out);
 
+/* Cholesky factor, in FP64 on the device (rocSOLVER), of A = G + (alpha +
+ * shift) I, with G the Gram of leviathan_features_gram (same rows, mean and
+ * terms 1 or 3), or, for terms = 0, the Gram formed in FP64 (rocBLAS DSYRK on
+ * the centred features in FP64), and shift the smallest of 0, 1e-7 s,
+ * 1e-6 s, ... (s = the mean of G's diagonal) for which A factorises.  The
+ * factor stays in the handle for leviathan_features_precond.  shift_out and
+ * gram_ms_out (the Gram's duration) may be NULL. */
+int leviathan_features_factor(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean, int terms,
+                              double alpha, double* shift_out, double* gram_ms_out);
+
+/* z = A^{-1} r (F doubles in and out, host memory) with the factor of the
+ * last leviathan_features_factor, on the device. */
+int leviathan_features_precond(LeviathanHandle* handle, const double* r, double* z);
+
 #ifdef __cplusplus
 }
 #endif

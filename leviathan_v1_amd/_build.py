@@ -64,7 +64,8 @@ def build_graph() -> str:
 def build_hip() -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = HIP_SO + ".tmp"
-    _run([_hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-o", tmp, *srcs])
+    # rocSOLVER (with rocBLAS) factorises the reservoir readout's FP64 Gram on the device
+    _run([_hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-o", tmp, *srcs, "-lrocsolver", "-lrocblas"])
     os.replace(tmp, HIP_SO)
     return HIP_SO
 

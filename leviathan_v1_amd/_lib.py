@@ -91,6 +91,8 @@ SIGNATURES = {
     "leviathan_features_colsum": (c_int, [vp, c_int64, c_int64, vp, vp]),
     "leviathan_features_matvec": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
     "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, c_int, vp, vp]),
+    "leviathan_features_factor": (c_int, [vp, c_int64, c_int64, vp, c_int, c_double, vp, vp]),
+    "leviathan_features_precond": (c_int, [vp, vp, vp]),
     # include/leviathan_phi.h
     "phi_init": (vp, [c_int, vp]),
     "phi_accumulate": (None, [vp, vp]),

@@ -4,6 +4,8 @@
 // (LeviathanHandle, leviathan_init/step/get_theta/set_theta/free/set_gain/
 // get_gain/set_gain_controller/get_vram_usage/reset_weights).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -144,6 +146,13 @@ struct LeviathanHandle {
   float* d_X = nullptr;
   int64_t* d_feat_base = nullptr;
   double* d_cs_partial = nullptr;  // kColsumChunks * 2 * feat_n
+  // readout: FP64 Cholesky factor of the last leviathan_features_factor (rocSOLVER)
+  rocblas_handle blas = nullptr;
+  double* d_chol = nullptr;       // [chol_n * chol_n], lower triangle
+  int32_t chol_n = 0;
+  double* d_vec = nullptr;        // [chol_n] right-hand side / solution
+  double* d_dmean = nullptr;      // mean of the Gram's diagonal
+  rocblas_int* d_info = nullptr;
   bool capturing = false;
 
   float* h_pinned = nullptr;    // kRHistSteps * n_nets floats
@@ -219,6 +228,10 @@ void destroy(LeviathanHandle* h) {
       if (p) (void)hipFree(p);
   }
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  void* rp[] = {h->d_chol, h->d_vec, h->d_dmean, h->d_info};
+  for (void* p : rp)
+    if (p) (void)hipFree(p);
+  if (h->blas) (void)rocblas_destroy_handle(h->blas);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -776,6 +789,55 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
 bool refresh_ctrl_field(LeviathanHandle* h, int net, size_t offset, const void* src, size_t bytes) {
   LEV_HIP(hipStreamSynchronize(h->stream));
   LEV_HIP(hipMemcpy(reinterpret_cast<char*>(h->d_ctrl + net) + offset, src, bytes, hipMemcpyHostToDevice));
+  return true;
+}
+
+}  // namespace
+
+namespace {
+
+// The FP32 matrix-core Gram of the centred feature rows [row0, row0 + nrows)
+// into a new device array *d_g (F x F, the caller frees it).
+bool gram_on_device(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, int terms, float** d_g,
+                    double* ms_out) {
+  const int32_t F = 2 * h->feat_n;
+  const int32_t Fp = (F + 255) / 256 * 256;  // whole 256-column Gram tiles
+  const int64_t Tp = (nrows + 63) / 64 * 64;
+  double* d_mean = nullptr;
+  uint16_t *d_xct = nullptr, *d_lo = nullptr;
+  float* g = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&d_mean, sizeof(double) * F);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_xct, sizeof(uint16_t) * (size_t)Fp * Tp);
+  if (e == hipSuccess && terms == 3) e = hipMalloc((void**)&d_lo, sizeof(uint16_t) * (size_t)Fp * Tp);
+  if (e == hipSuccess) e = hipMalloc((void**)&g, sizeof(float) * (size_t)F * F);
+  if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = lev::launch_center_bf16(h->d_X, F, row0, nrows, d_mean, F, Fp, d_xct, Tp, d_lo, h->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+  if (e == hipSuccess)
+    e = terms == 1 ? lev::launch_gram(d_xct, Tp, Fp, F, g, h->stream)
+                   : lev::launch_gram_split(d_xct, d_lo, Tp, Fp, F, g, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && ms_out) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, e0, e1);
+    *ms_out = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  void* ptrs[] = {d_mean, d_xct, d_lo};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) {
+    if (g) (void)hipFree(g);
+    set_err("features Gram: %s", hipGetErrorString(e));
+    return false;
+  }
+  *d_g = g;
   return true;
 }
 
@@ -1492,40 +1554,155 @@ int leviathan_features_gram(LeviathanHandle* h, int64_t row0, int64_t nrows, con
     return -1;
   }
   const int32_t F = 2 * h->feat_n;
-  const int32_t Fp = (F + 255) / 256 * 256;  // whole 256-column Gram tiles
-  const int64_t Tp = (nrows + 63) / 64 * 64;
-  double* d_mean = nullptr;
-  uint16_t *d_xct = nullptr, *d_lo = nullptr;
   float* d_g = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e = hipMalloc((void**)&d_mean, sizeof(double) * F);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_xct, sizeof(uint16_t) * (size_t)Fp * Tp);
-  if (e == hipSuccess && terms == 3) e = hipMalloc((void**)&d_lo, sizeof(uint16_t) * (size_t)Fp * Tp);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_g, sizeof(float) * (size_t)F * F);
-  if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = lev::launch_center_bf16(h->d_X, F, row0, nrows, d_mean, F, Fp, d_xct, Tp, d_lo, h->stream);
-  if (e == hipSuccess) e = hipEventCreate(&e0);
-  if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-  if (e == hipSuccess)
-    e = terms == 1 ? lev::launch_gram(d_xct, Tp, Fp, F, d_g, h->stream)
-                   : lev::launch_gram_split(d_xct, d_lo, Tp, Fp, F, d_g, h->stream);
-  if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  if (e == hipSuccess && ms_out) {
-    float ms = 0.f;
-    e = hipEventElapsedTime(&ms, e0, e1);
-    *ms_out = ms;
-  }
-  if (e == hipSuccess) e = hipMemcpy(gram_out, d_g, sizeof(float) * (size_t)F * F, hipMemcpyDeviceToHost);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  void* ptrs[] = {d_mean, d_xct, d_lo, d_g};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+  if (!gram_on_device(h, row0, nrows, mean, terms, &d_g, ms_out)) return -1;
+  const hipError_t e = hipMemcpy(gram_out, d_g, sizeof(float) * (size_t)F * F, hipMemcpyDeviceToHost);
+  (void)hipFree(d_g);
   if (e != hipSuccess) {
     set_err("features_gram: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_factor(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* mean, int terms,
+                              double alpha, double* shift_out, double* gram_ms_out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  if (!mean || (terms != 0 && terms != 1 && terms != 3) || !(alpha >= 0.0)) {
+    set_err("need the mean, terms 0, 1 or 3 and alpha >= 0");
+    return -1;
+  }
+  const int32_t F = 2 * h->feat_n;
+  if (!h->blas) {
+    if (rocblas_create_handle(&h->blas) != rocblas_status_success) {
+      h->blas = nullptr;
+      set_err("features_factor: rocblas_create_handle failed");
+      return -1;
+    }
+    (void)rocblas_set_stream(h->blas, h->stream);
+  }
+  if (h->chol_n != F) {
+    void* rp[] = {h->d_chol, h->d_vec, h->d_dmean, h->d_info};
+    for (void* p : rp)
+      if (p) (void)hipFree(p);
+    h->d_chol = nullptr;
+    h->d_vec = h->d_dmean = nullptr;
+    h->d_info = nullptr;
+    h->chol_n = 0;
+    hipError_t e = hipMalloc((void**)&h->d_chol, sizeof(double) * (size_t)F * F);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_vec, sizeof(double) * F);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_dmean, sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_info, sizeof(rocblas_int));
+    if (e != hipSuccess) {
+      set_err("features_factor: %s", hipGetErrorString(e));
+      return -1;
+    }
+    h->chol_n = F;
+  }
+  // the Gram in FP64: widened from the matrix-core Gram (terms 1, 3) or formed in FP64 (terms 0)
+  double* d_c = nullptr;
+  hipError_t e = hipMalloc((void**)&d_c, sizeof(double) * (size_t)F * F);
+  if (e != hipSuccess) {
+    set_err("features_factor: %s", hipGetErrorString(e));
+    return -1;
+  }
+  auto fail = [&](const char* what) {
+    (void)hipFree(d_c);
+    set_err("features_factor: %s", what);
+    return -1;
+  };
+  if (terms != 0) {
+    float* d_g = nullptr;
+    if (!gram_on_device(h, row0, nrows, mean, terms, &d_g, gram_ms_out)) {
+      (void)hipFree(d_c);
+      return -1;
+    }
+    e = lev::launch_widen(d_g, (int64_t)F * F, d_c, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_g);
+    if (e != hipSuccess) return fail(hipGetErrorString(e));
+  } else {
+    double* d_mean = nullptr;
+    double* d_xc = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    e = hipMalloc((void**)&d_mean, sizeof(double) * F);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_xc, sizeof(double) * (size_t)nrows * F);
+    if (e == hipSuccess) e = hipMemcpy(d_mean, mean, sizeof(double) * F, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = lev::launch_center_fp64(h->d_X, F, row0, nrows, d_mean, F, d_xc, h->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    bool blas_ok = e == hipSuccess;
+    if (blas_ok) {
+      // C = Xc^T Xc: Xc is [nrows][F] row-major, i.e. column-major F x nrows
+      const double one = 1.0, zero = 0.0;
+      blas_ok = rocblas_dsyrk(h->blas, rocblas_fill_lower, rocblas_operation_none, F, (rocblas_int)nrows, &one, d_xc,
+                              F, &zero, d_c, F) == rocblas_status_success;
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && gram_ms_out) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, e0, e1);
+      *gram_ms_out = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_mean) (void)hipFree(d_mean);
+    if (d_xc) (void)hipFree(d_xc);
+    if (e != hipSuccess) return fail(hipGetErrorString(e));
+    if (!blas_ok) return fail("rocblas_dsyrk failed");
+  }
+  double scale = 0.0;
+  e = lev::launch_diag_mean(d_c, F, h->d_dmean, h->stream);
+  if (e == hipSuccess) e = hipMemcpy(&scale, h->d_dmean, sizeof(double), hipMemcpyDeviceToHost);
+  if (!(scale > 0.0)) scale = 1.0;
+  double shift = 0.0;
+  bool done = false;
+  for (int attempt = 0; e == hipSuccess && attempt < 16 && !done; ++attempt) {
+    e = lev::launch_shifted(d_c, F, alpha + shift, h->d_chol, h->stream);
+    if (e != hipSuccess) break;
+    if (rocsolver_dpotrf(h->blas, rocblas_fill_lower, F, h->d_chol, F, h->d_info) != rocblas_status_success)
+      return fail("rocsolver_dpotrf failed");
+    rocblas_int info = 0;
+    e = hipMemcpy(&info, h->d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && info == 0) {
+      done = true;
+    } else {
+      shift = shift == 0.0 ? 1e-7 * scale : shift * 10.0;  // not positive definite: a larger shift
+    }
+  }
+  if (e != hipSuccess) return fail(hipGetErrorString(e));
+  (void)hipFree(d_c);
+  if (!done) {
+    set_err("features_factor: no diagonal shift up to %g made the Gram factorisable", shift);
+    return -1;
+  }
+  if (shift_out) *shift_out = shift;
+  return 0;
+}
+
+int leviathan_features_precond(LeviathanHandle* h, const double* r, double* z) {
+  if (!check_handle(h)) return -1;
+  if (!h->d_chol || h->chol_n != 2 * h->feat_n) {
+    set_err("features_precond: no factor (call leviathan_features_factor first)");
+    return -1;
+  }
+  if (!r || !z) {
+    set_err("features_precond: null vector");
+    return -1;
+  }
+  const int32_t F = h->chol_n;
+  hipError_t e = hipMemcpyAsync(h->d_vec, r, sizeof(double) * F, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess &&
+      rocsolver_dpotrs(h->blas, rocblas_fill_lower, F, 1, h->d_chol, F, h->d_vec, F) != rocblas_status_success) {
+    set_err("features_precond: rocsolver_dpotrs failed");
+    return -1;
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(z, h->d_vec, sizeof(double) * F, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_err("features_precond: %s", hipGetErrorString(e));
     return -1;
   }
   return 0;

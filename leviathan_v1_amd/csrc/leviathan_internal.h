@@ -157,6 +157,11 @@ hipError_t launch_center_bf16(const float* X, int64_t ld, int64_t row0, int64_t 
 hipError_t launch_gram(const uint16_t* XcT, int64_t Tp, int32_t Fp, int32_t F, float* G, hipStream_t s);
 hipError_t launch_gram_split(const uint16_t* Hi, const uint16_t* Lo, int64_t Tp, int32_t Fp, int32_t F, float* G,
                              hipStream_t s);
+hipError_t launch_widen(const float* G, int64_t n, double* C, hipStream_t s);
+hipError_t launch_shifted(const double* C, int32_t F, double diag_add, double* A, hipStream_t s);
+hipError_t launch_diag_mean(const double* C, int32_t F, double* diag_mean, hipStream_t s);
+hipError_t launch_center_fp64(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
+                              int32_t F, double* Xc, hipStream_t s);
 constexpr int kColsumChunks = 128;
 
 }  // namespace lev

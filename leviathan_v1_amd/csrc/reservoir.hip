@@ -14,11 +14,16 @@
 //                        (BF16 inputs, FP32 accumulation); 128x256 tiles on or
 //                        above the diagonal, LDS double buffering, mirrored.
 //
-// The host factorises G + alpha I once in FP64 and refines the solution with
-// residuals b - (Xc^T Xc + alpha I) w computed by the FP64 kernels above, so
-// the BF16 Gram only preconditions and the answer is the FP64 ridge solution.
+// The readout factorises G + alpha I once in FP64 on the device (rocSOLVER,
+// host code in leviathan_api.hip) and refines the solution with residuals
+// b - (Xc^T Xc + alpha I) w computed by the FP64 kernels above, so the BF16
+// Gram only preconditions and the answer is the FP64 ridge solution.  For
+// features too ill-conditioned for that, the Gram is formed in FP64
+// (center_fp64_kernel, then rocBLAS DSYRK) instead.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "leviathan_internal.h"
 #include "step_kernel.h"
@@ -330,6 +335,68 @@ hipError_t launch_gram(const uint16_t* XcT, int64_t Tp, int32_t Fp, int32_t F, f
   int32_t tiles = 0;
   for (int32_t I = 0; I < nI; ++I) tiles += nJ - I / 2;
   gram_kernel<<<dim3(tiles), dim3(256), 0, s>>>(XcT, Tp, nI, nJ, F, G);
+  return hipGetLastError();
+}
+
+// C = G widened to FP64.
+__global__ __launch_bounds__(256) void widen_kernel(const float* G, int64_t n, double* C) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    C[q] = (double)G[q];
+}
+
+// A = C + diag_add I: the matrix the readout factorises.
+__global__ __launch_bounds__(256) void shifted_kernel(const double* C, int32_t F, double diag_add, double* A) {
+  const int64_t n = (int64_t)F * F;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = (int32_t)(q / F), j = (int32_t)(q % F);
+    A[q] = C[q] + (i == j ? diag_add : 0.0);
+  }
+}
+
+// The mean of C's diagonal (one workgroup, a fixed order).
+__global__ __launch_bounds__(256) void diag_mean_kernel(const double* C, int32_t F, double* diag_mean) {
+  __shared__ double s_part[256 / 64];
+  double acc = 0.0;
+  for (int32_t i = threadIdx.x; i < F; i += 256) acc += C[(int64_t)i * F + i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *diag_mean = (s_part[0] + s_part[1] + s_part[2] + s_part[3]) / (double)F;
+}
+
+// Xc = X[row0 : row0 + nrows] - mean in FP64, [nrows][F] row-major (for the
+// FP64 Gram: column-major F x nrows with leading dimension F).
+__global__ __launch_bounds__(256) void center_fp64_kernel(const float* __restrict__ X, int64_t ld, int64_t row0,
+                                                          int64_t nrows, const double* __restrict__ mean, int32_t F,
+                                                          double* __restrict__ Xc) {
+  const int64_t n = nrows * F;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = q / F;
+    const int32_t f = (int32_t)(q % F);
+    Xc[q] = (double)X[(row0 + t) * ld + f] - mean[f];
+  }
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
+
+hipError_t launch_widen(const float* G, int64_t n, double* C, hipStream_t s) {
+  widen_kernel<<<grid_for(n), 256, 0, s>>>(G, n, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_shifted(const double* C, int32_t F, double diag_add, double* A, hipStream_t s) {
+  shifted_kernel<<<grid_for((int64_t)F * F), 256, 0, s>>>(C, F, diag_add, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_mean(const double* C, int32_t F, double* diag_mean, hipStream_t s) {
+  diag_mean_kernel<<<1, 256, 0, s>>>(C, F, diag_mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_center_fp64(const float* X, int64_t ld, int64_t row0, int64_t nrows, const double* mean,
+                              int32_t F, double* Xc, hipStream_t s) {
+  center_fp64_kernel<<<grid_for(nrows * F), 256, 0, s>>>(X, ld, row0, nrows, mean, F, Xc);
   return hipGetLastError();
 }
 

@@ -314,6 +314,27 @@ class LeviathanEngine:
             _lib.check("features_gram")
         return g, ms.value
 
+    def features_factor(self, row0: int, nrows: int, mean, alpha: float, terms: int = 1):
+        """FP64 Cholesky factor of Gram + (alpha + shift) I on the device, kept for features_precond;
+        terms 1 or 3: the matrix-core Gram (as features_gram), 0: the Gram formed in FP64.
+        Returns (shift, Gram ms)."""
+        F = 2 * self.n_feat_nodes
+        mean = _arr(mean, np.float64, F, "mean")
+        shift = ctypes.c_double(0.0)
+        ms = ctypes.c_double(0.0)
+        if _lib.lib().leviathan_features_factor(self._h, int(row0), int(nrows), _p(mean), int(terms), float(alpha),
+                                                ctypes.byref(shift), ctypes.byref(ms)) != 0:
+            _lib.check("features_factor")
+        return shift.value, ms.value
+
+    def features_precond(self, r) -> np.ndarray:
+        """(Gram + (alpha + shift) I)^{-1} r with the factor of the last features_factor."""
+        r = _arr(r, np.float64, 2 * self.n_feat_nodes, "r")
+        z = np.empty_like(r)
+        if _lib.lib().leviathan_features_precond(self._h, _p(r), _p(z)) != 0:
+            _lib.check("features_precond")
+        return z
+
     def set_gain_net(self, net: int, g: float) -> None:
         if _lib.lib().leviathan_set_gain_net(self._h, int(net), float(g)) != 0:
             _lib.check("set_gain_net")

@@ -76,36 +76,21 @@ class LeviathanReservoir:
         return g + alpha * x
 
     def _factor(self, row0, nrows, mean, alpha, terms):
-        """Cholesky factor of the matrix-core Gram + alpha I, with the smallest diagonal shift that
-        makes it factorisable."""
-        import scipy.linalg as sla
+        """Cholesky factor, on the device, of the matrix-core Gram + alpha I with the smallest diagonal
+        shift that makes it factorisable; returns (shift, Gram kernel ms)."""
+        return self.engine.features_factor(row0, nrows, mean, alpha, terms=terms)
 
-        t0 = time.perf_counter()
-        G, gram_ms = self.engine.features_gram(row0, nrows, mean, terms=terms)
-        gram_s = time.perf_counter() - t0
-        M = G.astype(np.float64)
-        scale = float(np.mean(np.diag(M))) or 1.0
-        shift = 0.0
-        while True:
-            try:
-                Mi = M.copy()
-                Mi[np.diag_indices_from(Mi)] += alpha + shift
-                return sla.cho_factor(Mi, lower=True, check_finite=False), shift, gram_ms, gram_s
-            except np.linalg.LinAlgError:
-                shift = max(shift * 10.0, 1e-7 * scale)
-
-    def _pcg(self, cf, b, x, row0, nrows, mean, alpha, tol, max_iter, stall):
-        """Preconditioned conjugate gradients on the exact FP64 operator from x; stops at tol, at
-        max_iter, or when the residual fell by less than 2x over the last ``stall`` iterations."""
-        import scipy.linalg as sla
-
+    def _pcg(self, b, x, row0, nrows, mean, alpha, tol, max_iter, stall):
+        """Preconditioned conjugate gradients on the exact FP64 operator from x, preconditioned by the
+        device factor; stops at tol, at max_iter, or when the residual fell by less than 2x over the
+        last ``stall`` iterations."""
         bn = float(np.linalg.norm(b)) or 1.0
         r = b - self._A(row0, nrows, mean, alpha, x)
         rel = float(np.linalg.norm(r)) / bn
         hist = [rel]
         if rel <= tol:
             return x, 0, rel
-        z = sla.cho_solve(cf, r, check_finite=False)
+        z = self.engine.features_precond(r)
         p = z.copy()
         rz = float(r @ z)
         it = 0
@@ -119,7 +104,7 @@ class LeviathanReservoir:
             hist.append(rel)
             if rel <= tol or (stall and it >= stall and rel > 0.5 * hist[-1 - stall]):
                 break
-            z = sla.cho_solve(cf, r, check_finite=False)
+            z = self.engine.features_precond(r)
             rz_new = float(r @ z)
             p = z + (rz_new / rz) * p
             rz = rz_new
@@ -129,10 +114,9 @@ class LeviathanReservoir:
             gram_terms=None) -> RidgeResult:
         """Ridge regression with intercept on rows [row0, row0 + len(target)).
 
-        gram_terms: 1 (BF16 Gram), 3 (split hi/lo BF16 Gram) or None: start from the BF16 Gram and
-        switch to the split Gram when the refinement stalls (ill-conditioned features)."""
-        import scipy.linalg as sla
-
+        gram_terms: 1 (BF16 Gram), 3 (split hi/lo BF16 Gram), 0 (FP64 Gram) or None: start from the
+        BF16 Gram and switch to the split Gram, then to the FP64 Gram, when the refinement stalls
+        (ill-conditioned features)."""
         y = np.asarray(target, dtype=np.float64)
         nrows = y.shape[0]
         t0 = time.perf_counter()
@@ -141,25 +125,24 @@ class LeviathanReservoir:
         yc = y - ybar
         b = self.engine.features_colsum(row0, nrows, yc) - mean * yc.sum()
         t1 = time.perf_counter()
-        plan = [gram_terms] if gram_terms else [1, 3]
-        timings = {"moments_s": t1 - t0, "gram_s": 0.0, "factor_s": 0.0, "refine_s": 0.0}
+        plan = [gram_terms] if gram_terms is not None else [1, 3, 0]
+        timings = {"moments_s": t1 - t0, "factor_s": 0.0, "refine_s": 0.0}
         x = None
         its = 0
+        shift = 0.0
         for k, terms in enumerate(plan):
             t2 = time.perf_counter()
-            cf, shift, gram_ms, gram_s = self._factor(row0, nrows, mean, alpha, terms)
+            shift, gram_ms = self._factor(row0, nrows, mean, alpha, terms)
             t3 = time.perf_counter()
             if x is None:
-                x = sla.cho_solve(cf, b, check_finite=False)
+                x = self.engine.features_precond(b)
             last = k == len(plan) - 1
-            x, it, rel = self._pcg(cf, b, x, row0, nrows, mean, alpha, tol, max_iter - its,
-                                   0 if last else 25)
+            x, it, rel = self._pcg(b, x, row0, nrows, mean, alpha, tol, max_iter - its, 0 if last else 25)
             its += it
             t4 = time.perf_counter()
-            timings["gram_s"] += gram_s
-            timings["factor_s"] += t3 - t2 - gram_s
+            timings["factor_s"] += t3 - t2  # Gram on the matrix cores + FP64 Cholesky on the device
             timings["refine_s"] += t4 - t3
-            timings["gram_kernel_ms"] = gram_ms
+            timings.setdefault("gram_ms_by_terms", {})[terms] = gram_ms
             timings["gram_terms"] = terms
             if rel <= tol:
                 break

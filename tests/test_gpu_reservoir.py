@@ -109,6 +109,37 @@ def test_split_gram_matches_fp64_gram_of_hi_lo_parts(res):
     assert err3 < 2e-5 and err3 < err1 / 20
 
 
+@pytest.mark.parametrize("terms", [1, 0])
+@pytest.mark.parametrize("alpha", [1e-4, 1.0])
+def test_device_cholesky_solves_the_shifted_gram(res, alpha, terms):
+    """features_factor / features_precond: z solves (G + (alpha + shift) I) z = r to FP64 accuracy,
+    with G the matrix-core Gram features_gram returns for the same rows (terms 1) or the FP64 Gram
+    of the centred features (terms 0), F = 300; the shift is the smallest in 0, 1e-7 s, 1e-6 s, ...
+    that factorises (numpy's Cholesky agrees)."""
+    say("device cholesky", alpha, terms)
+    r = res["res"]
+    row0, n = 300, 2900
+    X = r.features(row0, n).astype(np.float64)
+    mean = X.mean(axis=0)
+    if terms == 1:
+        G, _ = r.engine.features_gram(row0, n, mean)
+        G = G.astype(np.float64)
+    else:
+        G = (X - mean).T @ (X - mean)
+    shift, ms = r.engine.features_factor(row0, n, mean, alpha, terms=terms)
+    A = G + (alpha + shift) * np.eye(G.shape[0])
+    np.linalg.cholesky(A)  # factorisable at the shift found
+    if shift > 0:
+        with pytest.raises(np.linalg.LinAlgError):
+            np.linalg.cholesky(G + (alpha + shift / 10) * np.eye(G.shape[0]))
+    rhs = np.random.default_rng(9).standard_normal(G.shape[0])
+    z = r.engine.features_precond(rhs)
+    ref = np.linalg.solve(A, rhs)
+    cond = np.linalg.cond(A)
+    assert np.linalg.norm(A @ z - rhs) / np.linalg.norm(rhs) < 1e-13 * cond
+    assert np.abs(z - ref).max() / np.abs(ref).max() < 1e-14 * cond
+
+
 def test_ridge_fit_matches_fp64_oracle(res):
     """Preconditioned FP64 refinement reaches the FP64 ridge solution of the same features."""
     say("ridge fit")
@@ -116,7 +147,7 @@ def test_ridge_fit_matches_fp64_oracle(res):
     lag, row0 = 10, 200
     n = 3200 - row0 - lag
     y = u[row0 + lag: row0 + lag + n].astype(np.float64)
-    for alpha, terms in ((1e-4, None), (1.0, 1), (1e-4, 3)):
+    for alpha, terms in ((1e-4, None), (1.0, 1), (1e-4, 3), (1e-4, 0)):
         fit = r.fit(y, row0, alpha=alpha, tol=1e-12, gram_terms=terms)
         X = r.features(row0, n)
         w_ref, c_ref = oracle.ridge_fp64(X, y, alpha)
