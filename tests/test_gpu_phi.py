@@ -154,6 +154,35 @@ def test_attached_engine_records_every_step(history, schedule):
     b.close()
 
 
+def test_attached_ensemble_engine():
+    """An ensemble of 40 networks (its finalize is a grid of workgroups): hubs in several networks
+    record exactly what the oracle records from the phases after every step."""
+    from leviathan_v1_amd import LeviathanEngine, graph, workloads
+
+    say("phi attached ensemble")
+    wls = [workloads.watts_strogatz_workload(300 + 20 * k, 20, 0.1, 50, seed=40 + k) for k in range(40)]
+    g, offs = graph.disjoint_union([w.graph for w in wls])
+    cat = lambda f: np.concatenate([getattr(w, f) for w in wls])
+    args = (g.num_nodes, 50, g.row_ptr, g.col_idx, cat("delays"), cat("weights"), cat("theta"), cat("theta_hat"),
+            cat("omega"))
+    kw = dict(n_nets=40, net_offsets=offs, g0=list(np.linspace(0, 4, 40)), controller=[False] * 40)
+    a = LeviathanEngine(*args, **kw)
+    b = LeviathanEngine(*args, **kw)
+    hubs = [int(offs[k]) + 7 for k in (0, 9, 17, 25, 33, 39)]
+    w = _worker(hubs)
+    w.attach(a)
+    o = P.PhiOracle(hubs)
+    a.step_n(DT, 150)
+    for _ in range(150):
+        b.step(DT)
+        o.accumulate(b.get_theta())
+    assert np.array_equal(a.get_theta(), b.get_theta())
+    _check_stats(w, o)
+    w.close()
+    a.close()
+    b.close()
+
+
 def test_attach_rules_and_free_while_attached():
     say("phi attach rules")
     wl = _wl(500, seed=3)
