@@ -262,8 +262,10 @@ int leviathan_features_gram(LeviathanHandle* handle, int64_t row0, int64_t nrows
 int leviathan_features_factor(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean, int terms,
                               double alpha, double* shift_out, double* gram_ms_out);
 
-/* z = A^{-1} r (F doubles in and out, host memory) with the factor of the
- * last leviathan_features_factor, on the device. */
+/* z = A^{-1} r (F doubles in and out, host memory) with the factorisation of
+ * the last leviathan_features_factor, on the device: one DSYMV with A^{-1}
+ * (formed from the Cholesky factor by rocSOLVER dpotri) for the matrix-core
+ * Grams, two triangular solves (dpotrs) for the FP64 Gram. */
 int leviathan_features_precond(LeviathanHandle* handle, const double* r, double* z);
 
 #ifdef __cplusplus

@@ -150,7 +150,8 @@ struct LeviathanHandle {
   rocblas_handle blas = nullptr;
   double* d_chol = nullptr;       // [chol_n * chol_n], lower triangle
   int32_t chol_n = 0;
-  double* d_vec = nullptr;        // [chol_n] right-hand side / solution
+  double* d_vec = nullptr;        // [2 * chol_n] right-hand side, solution
+  bool chol_inverse = false;      // d_chol holds A^{-1} (matrix-core Grams), else the factor (FP64 Gram)
   double* d_dmean = nullptr;      // mean of the Gram's diagonal
   rocblas_int* d_info = nullptr;
   bool capturing = false;
@@ -1590,7 +1591,7 @@ int leviathan_features_factor(LeviathanHandle* h, int64_t row0, int64_t nrows, c
     h->d_info = nullptr;
     h->chol_n = 0;
     hipError_t e = hipMalloc((void**)&h->d_chol, sizeof(double) * (size_t)F * F);
-    if (e == hipSuccess) e = hipMalloc((void**)&h->d_vec, sizeof(double) * F);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_vec, sizeof(double) * 2 * F);
     if (e == hipSuccess) e = hipMalloc((void**)&h->d_dmean, sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&h->d_info, sizeof(rocblas_int));
     if (e != hipSuccess) {
@@ -1667,6 +1668,27 @@ int leviathan_features_factor(LeviathanHandle* h, int64_t row0, int64_t nrows, c
     rocblas_int info = 0;
     e = hipMemcpy(&info, h->d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost);
     if (e == hipSuccess && info == 0) {
+      // For the approximate (matrix-core) Grams, the inverse from the factor: each
+      // preconditioner application is then one bandwidth-bound DSYMV rather than two
+      // latency-bound triangular solves.  The FP64 Gram serves ill-conditioned features,
+      // where the explicit inverse would lose digits: it keeps the triangular solves.
+      h->chol_inverse = terms != 0;
+      if (h->chol_inverse) {
+        if (rocsolver_dpotri(h->blas, rocblas_fill_lower, F, h->d_chol, F, h->d_info) != rocblas_status_success)
+          return fail("rocsolver_dpotri failed");
+        e = hipMemcpy(&info, h->d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && info != 0) {
+          // the inverse of a nearly singular factor did not form: factorise again and keep the
+          // triangular solves
+          h->chol_inverse = false;
+          e = lev::launch_shifted(d_c, F, alpha + shift, h->d_chol, h->stream);
+          if (e == hipSuccess &&
+              rocsolver_dpotrf(h->blas, rocblas_fill_lower, F, h->d_chol, F, h->d_info) != rocblas_status_success)
+            return fail("rocsolver_dpotrf failed");
+          if (e == hipSuccess) e = hipMemcpy(&info, h->d_info, sizeof(rocblas_int), hipMemcpyDeviceToHost);
+          if (e == hipSuccess && info != 0) return fail("the factorisation did not repeat");
+        }
+      }
       done = true;
     } else {
       shift = shift == 0.0 ? 1e-7 * scale : shift * 10.0;  // not positive definite: a larger shift
@@ -1694,12 +1716,22 @@ int leviathan_features_precond(LeviathanHandle* h, const double* r, double* z) {
   }
   const int32_t F = h->chol_n;
   hipError_t e = hipMemcpyAsync(h->d_vec, r, sizeof(double) * F, hipMemcpyHostToDevice, h->stream);
-  if (e == hipSuccess &&
-      rocsolver_dpotrs(h->blas, rocblas_fill_lower, F, 1, h->d_chol, F, h->d_vec, F) != rocblas_status_success) {
-    set_err("features_precond: rocsolver_dpotrs failed");
-    return -1;
+  const double one = 1.0, zero = 0.0;
+  if (h->chol_inverse) {
+    if (e == hipSuccess && rocblas_dsymv(h->blas, rocblas_fill_lower, F, &one, h->d_chol, F, h->d_vec, 1, &zero,
+                                         h->d_vec + F, 1) != rocblas_status_success) {
+      set_err("features_precond: rocblas_dsymv failed");
+      return -1;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(z, h->d_vec + F, sizeof(double) * F, hipMemcpyDeviceToHost, h->stream);
+  } else {
+    if (e == hipSuccess &&
+        rocsolver_dpotrs(h->blas, rocblas_fill_lower, F, 1, h->d_chol, F, h->d_vec, F) != rocblas_status_success) {
+      set_err("features_precond: rocsolver_dpotrs failed");
+      return -1;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(z, h->d_vec, sizeof(double) * F, hipMemcpyDeviceToHost, h->stream);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(z, h->d_vec, sizeof(double) * F, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
     set_err("features_precond: %s", hipGetErrorString(e));

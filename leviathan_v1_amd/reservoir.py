@@ -146,8 +146,19 @@ class LeviathanReservoir:
             timings["gram_terms"] = terms
             if rel <= tol:
                 break
-        # true residual of the final iterate
+        # true residual of the final iterate; the recursive residual drifts from it on ill-conditioned
+        # features, so restart from the iterate (which recomputes the residual exactly) while it is
+        # above tol and iterations remain
         rel = float(np.linalg.norm(b - self._A(row0, nrows, mean, alpha, x))) / (float(np.linalg.norm(b)) or 1.0)
+        restarts = 0
+        while rel > tol and its < max_iter and restarts < 3:
+            t3 = time.perf_counter()
+            x, it, _ = self._pcg(b, x, row0, nrows, mean, alpha, tol, max_iter - its, 0)
+            its += it
+            restarts += 1
+            timings["refine_s"] += time.perf_counter() - t3
+            rel = float(np.linalg.norm(b - self._A(row0, nrows, mean, alpha, x))) / (float(np.linalg.norm(b)) or 1.0)
+        timings["restarts"] = restarts
         intercept = ybar - float(mean @ x)
         return RidgeResult(x, intercept, its, rel, shift, timings)
 
