@@ -63,7 +63,8 @@ def make_workload(name, n_override=None):
     raise ValueError(name)
 
 
-def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", bucket_edges=0, profile_steps=20):
+def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", bucket_edges=0, profile_steps=20,
+             scatter_steps="auto"):
     from leviathan_v1_amd import LeviathanEngine
 
     t0 = time.perf_counter()
@@ -74,6 +75,14 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         tuned = eng.autotune(wl.dt, steps=5) if schedule == "auto" else None
         chosen = eng.get_schedule()
         log(f"{history}: autotune {tuned} -> {chosen}")
+        # steps per scatter pass (two-phase schedules; the direct one has no scatter)
+        k_ms = None
+        if chosen[0] != "direct":
+            if scatter_steps == "auto":
+                k_ms = eng.autotune_scatter_steps(wl.dt, steps=8)
+            else:
+                eng.set_scatter_steps(int(scatter_steps))
+            log(f"{history}: scatter steps {k_ms} -> {eng.scatter_steps}")
         eng.step_n(wl.dt, warmup)
         t0 = time.perf_counter()
         r = eng.step_n(wl.dt, steps)  # returns after the stream is synchronised
@@ -83,7 +92,8 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         ok = bool(np.isfinite(theta).all() and np.isfinite(r).all() and 0 <= r.min() and r.max() <= 1)
         return {
             "history": history, "layout": layout, "schedule": eng.schedule_name,
-            "autotune_ms_per_step": tuned,
+            "autotune_ms_per_step": tuned, "scatter_steps": eng.scatter_steps if chosen[0] != "direct" else None,
+            "scatter_steps_ms_per_step": k_ms,
             "seconds": el, "ms_per_step": 1e3 * el / steps,
             "edge_steps_per_s": wl.E * steps / el,
             "hbm_gbs": algorithmic_bytes(wl.N, wl.E, history) * steps / el / 1e9,
@@ -136,27 +146,40 @@ def copy_calibration(nbytes: int, reps: int = 3) -> None:
     log(f"copy calibration: {reps} x {nbytes} bytes device-to-device")
 
 
-def traffic_key(workload, history, layout, schedule):
+def traffic_key(workload, history, layout, schedule, scatter_steps=None):
     """Key of a workload's record in profiles/traffic.json."""
-    return f"{workload}:{history}:{schedule}" if layout == "node" else f"{workload}:{history}:{layout}:{schedule}"
+    key = f"{workload}:{history}:{schedule}" if layout == "node" else f"{workload}:{history}:{layout}:{schedule}"
+    return key + (f":k{scatter_steps}" if scatter_steps and scatter_steps > 1 else "")
 
 
-def load_traffic(workload, history, layout, schedule):
+def load_traffic(workload, history, layout, schedule, scatter_steps=None):
     """HBM bytes per step of the step's kernels, from the rocprofv3 counter runs
-    summarised in profiles/traffic.json (None if that workload was not profiled)."""
+    summarised in profiles/traffic.json (None if that workload was not profiled).
+    With K steps per scatter pass, a step carries 1/K of the scatter pass and of
+    its block's early-edge gathers."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            rec = json.load(f).get(traffic_key(workload, history, layout, schedule))
+            rec = json.load(f).get(traffic_key(workload, history, layout, schedule, scatter_steps))
     except (OSError, ValueError):
         return None
     if not rec:
         return None
-    ks = {k: v for k, v in rec["kernels"].items()
-          if "step_kernel" in k or "scatter_kernel" in k or "regroup_kernel" in k}
-    return sum(v["fetch_bytes"] + v["write_bytes"] for v in ks.values()) or None
+    byt = lambda v: v["fetch_bytes"] + v["write_bytes"]
+    kern = rec["kernels"]
+    total = sum(byt(v) for k, v in kern.items() if "step_kernel" in k or "regroup_kernel" in k)
+    K = scatter_steps if scatter_steps and scatter_steps > 1 else 1
+    scat = [(k, v) for k, v in kern.items() if "scatter_kernel" in k and (K == 1 or k.endswith(f", {K}>"))]
+    if scat:
+        k, v = scat[0]
+        per_block = byt(v)
+        early = [e for n, e in kern.items() if "early_kernel" in n]
+        if K > 1 and early:
+            per_block += byt(early[0]) * early[0]["dispatches"] / v["dispatches"]
+        total += per_block / K
+    return total or None
 
 
 def main():
@@ -173,6 +196,8 @@ def main():
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned", "staged"])
     ap.add_argument("--bucket-edges", type=int, default=0)
+    ap.add_argument("--scatter-steps", default="auto", choices=["auto", "1", "2", "4"],
+                    help="steps per scatter pass of the two-phase schedules (default: timed and chosen)")
     ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
@@ -215,15 +240,19 @@ def main():
         else:
             hist = "f32" if f == "f32-tuned" else f
             forms[f] = run_form(wl, hist, args.steps, args.warmup, args.layout, args.ept, args.schedule,
-                                args.bucket_edges)
+                                args.bucket_edges, scatter_steps=args.scatter_steps)
         log(f, json.dumps(forms[f]))
     head = forms.get("u16") or forms.get("f32-tuned") or forms["f32"]
     hist = head["history"]
     alg = algorithmic_bytes(wl.N, wl.E, hist)
-    # the step's kernels: the fused step kernel (direct) or scatter + step kernels (two-phase schedules)
+    # the step's kernels: the fused step kernel (direct) or scatter + step kernels (two-phase schedules;
+    # with several steps per scatter pass the scatter time is that of the pass and the early-edge
+    # gathers, per step)
     step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"] + head["regroup_kernel_ms"]
     kernels = {"direct": "step_kernel", "binned": "scatter_kernel+step_kernel",
                "staged": "scatter_kernel+regroup_kernel+step_kernel"}[head["schedule"].split("/")[0]]
+    if (head.get("scatter_steps") or 1) > 1:
+        kernels = kernels.replace("scatter_kernel", f"(scatter_kernel+early_kernel)/{head['scatter_steps']}")
     achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -242,7 +271,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded graph and initial state)",
         "config": {"workload": wl.name, "N": wl.N, "E": wl.E, "max_delay": wl.max_delay,
-                   "history": hist, "layout": head["layout"], "schedule": head["schedule"]},
+                   "history": hist, "layout": head["layout"], "schedule": head["schedule"],
+                   "scatter_steps": head.get("scatter_steps")},
         "hbm_gbs": head["hbm_gbs"],
         "plain_fp32": forms.get("f32"),
         "fp32_tuned": forms.get("f32-tuned"),
@@ -253,7 +283,8 @@ def main():
                                       if "u16" in forms and "f32-tuned" in forms else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic(args.workload, hist, head["layout"], head["schedule"]),
+                     "traffic": load_traffic(args.workload, hist, head["layout"], head["schedule"],
+                                             head.get("scatter_steps")),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }

@@ -182,7 +182,8 @@ int leviathan_num_blocks(LeviathanHandle* handle);
 
 /* Runs n eager steps and times them with HIP events on the handle's stream.
  * out[0] = mean duration of the row pass, the step kernel (ms per launch),
- * out[1] = mean duration of the per-step finalize kernel (ms per launch),
+ * out[1] = mean duration of the per-step finalize (chunk_sum_kernel, launched
+ *          for a single large graph, plus finalize_kernel; ms per step),
  * out[2] = wall time of the n steps between the first and last event (ms),
  * out[3] = mean duration of the scatter kernel (two-phase schedules; 0 otherwise),
  * out[4] = mean duration of the regroup kernel (staged schedule; 0 otherwise).
@@ -203,6 +204,22 @@ int leviathan_profile_steps(LeviathanHandle* handle, float dt, int n, double* ou
 int leviathan_set_schedule(LeviathanHandle* handle, int schedule, int edges_per_thread);
 int leviathan_get_schedule(LeviathanHandle* handle, int* schedule, int* edges_per_thread);
 int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_out);
+
+/* Multi-step scatter of the two-phase schedules.  With steps = K > 1, the
+ * steps run by leviathan_step_n (in whole blocks of K; a remainder, and
+ * leviathan_step, run singly) share one scatter pass: the first step of a
+ * block gathers the delayed phases of all K steps, and step k of the block
+ * first gathers only its edges with delay < k (their phases do not exist
+ * before).  Phases, weights and r are bit-identical to K = 1.  K is 1, 2 or
+ * 4; the handle keeps K bucketed value arrays (K * E values).
+ * leviathan_get_scatter_steps returns K or -1.
+ * leviathan_autotune_scatter_steps times K = 1, 2, 4 on the active plan from
+ * the same state (ms per step to ms_out[3]), restores the state and keeps the
+ * fastest; with the direct schedule it changes nothing and writes NaN.
+ * Return 0 or -1.  (No counterpart in the reference.) */
+int leviathan_set_scatter_steps(LeviathanHandle* handle, int steps);
+int leviathan_get_scatter_steps(LeviathanHandle* handle);
+int leviathan_autotune_scatter_steps(LeviathanHandle* handle, float dt, int steps, double* ms_out);
 
 /* Input drive (reservoir): omega_i(t) = omega_i + drive_b[i] * u_t.
  * leviathan_set_drive uploads drive_b[N] (NULL switches the drive off);

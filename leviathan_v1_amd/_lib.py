@@ -83,6 +83,9 @@ SIGNATURES = {
     "leviathan_set_schedule": (c_int, [vp, c_int, c_int]),
     "leviathan_get_schedule": (c_int, [vp, vp, vp]),
     "leviathan_autotune": (c_int, [vp, c_float, c_int, vp]),
+    "leviathan_set_scatter_steps": (c_int, [vp, c_int]),
+    "leviathan_get_scatter_steps": (c_int, [vp]),
+    "leviathan_autotune_scatter_steps": (c_int, [vp, c_float, c_int, vp]),
     "leviathan_step_n_driven": (c_int, [vp, c_float, c_int, vp, vp]),
     "leviathan_features_setup": (c_int, [vp, vp, c_int, c_int]),
     "leviathan_step_n_features": (c_int, [vp, c_float, c_int, vp, vp]),

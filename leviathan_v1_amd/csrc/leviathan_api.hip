@@ -80,6 +80,7 @@ struct DevPlan {
   uint16_t* d_rg_rank = nullptr;
   int2* d_rg_runs = nullptr;
   int32_t n_rg_chunks = 0;
+  uint32_t* d_early_pos = nullptr;  // two-phase plans: value slot of each early edge (multi-step scatter)
   bool owns_pos = false;
 };
 
@@ -127,6 +128,13 @@ struct LeviathanHandle {
   uint16_t* d_run_idx = nullptr;  // binned plans' scatter runs
   int32_t* d_run_ptr = nullptr;
   int32_t* d_run_base = nullptr;
+  // multi-step scatter: the edges with delay < kMaxScatterSteps - 1 by (delay, source)
+  int32_t* d_early_src = nullptr;
+  uint8_t* d_early_delay = nullptr;
+  uint32_t* d_early_pos = nullptr;  // binned plans' value slots of those edges
+  int32_t early_cnt[lev::kMaxScatterSteps] = {};  // early_cnt[k] = edges with delay < k
+  int scatter_steps = 1;   // consecutive steps one scatter pass gathers (two-phase schedules)
+  int hbuf_steps = 1;      // steps d_hbuf has room for
   int4* d_sources = nullptr;  // phase-1 work items
   int32_t n_sources = 0;
   int32_t n_buckets = 0;
@@ -213,13 +221,14 @@ void destroy(LeviathanHandle* h) {
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_fin_done, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_dsrc,
                   h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_run_idx, h->d_run_ptr, h->d_run_base, h->d_sources,  h->d_feat_nodes,
+                  h->d_early_src, h->d_early_delay, h->d_early_pos,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
     if (pl.owns_pos) {
       void* pp[] = {pl.d_pos,   pl.d_rank,      pl.d_run_idx,  pl.d_run_ptr, pl.d_run_base,
-                    pl.d_lpos,  pl.d_rg_chunks, pl.d_rg_rank,  pl.d_rg_runs};
+                    pl.d_lpos,  pl.d_rg_chunks, pl.d_rg_rank,  pl.d_rg_runs, pl.d_early_pos};
       for (void* p : pp)
         if (p) (void)hipFree(p);
     }
@@ -319,43 +328,88 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   sa.xcd_remap = 1;
   sa.blocks = h->d_sources;
   sa.hbuf = h->d_hbuf;
+  sa.hbuf_stride = h->E;
+  sa.steps = 1;
   sa.step = h->d_step;
   return sa;
 }
 
-// Enqueue the step kernel(s) of the active plan, without the finalize.
-// mid / mid2 (optional) are recorded after the scatter and after the regroup.
-hipError_t enqueue_main(LeviathanHandle* h, float dt, hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr) {
+// Steps one scatter pass gathers under the active plan (1 for the direct schedule).
+int block_steps(const LeviathanHandle* h) {
+  return h->plans[h->active].kind == LEV_SCHEDULE_DIRECT ? 1 : h->scatter_steps;
+}
+
+// Enqueue the step kernel(s) of the active plan, without the finalize, for
+// step k of a block of K consecutive steps whose values one scatter pass
+// gathers (K = 1: a step on its own).  Step 0 of a block runs the scatter
+// for all K steps; step k > 0 first gathers its edges with delay < k.
+// mid / mid2 (optional) are recorded after the scatter phase and after the regroup.
+hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t mid = nullptr,
+                        hipEvent_t mid2 = nullptr) {
   const DevPlan& pl = h->plans[h->active];
   if (pl.kind == LEV_SCHEDULE_DIRECT)
     return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
-  hipError_t e = lev::launch_scatter(h->cfg.history, h->cfg.layout, scatter_args(h, pl), h->stream);
+  const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+  void* hk = static_cast<char*>(h->d_hbuf) + (size_t)k * (size_t)h->E * hist_bytes;
+  hipError_t e = hipSuccess;
+  if (k == 0) {
+    lev::ScatterArgs sa = scatter_args(h, pl);
+    sa.steps = K;
+    e = lev::launch_scatter(h->cfg.history, h->cfg.layout, sa, h->stream);
+  } else {
+    lev::EarlyArgs ea{};
+    ea.src = h->d_early_src;
+    ea.pos = pl.d_early_pos;
+    ea.delay = h->d_early_delay;
+    ea.count = h->early_cnt[k];
+    ea.ring = h->d_ring;
+    ea.ring_pitch = h->ring_pitch;
+    ea.n_slots = h->n_slots;
+    ea.hbuf = hk;
+    ea.step = h->d_step;
+    e = lev::launch_early(h->cfg.history, h->cfg.layout, ea, h->stream);
+  }
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
   if (e == hipSuccess && pl.kind == LEV_SCHEDULE_STAGED) {
     lev::RegroupArgs ra{};
     ra.chunks = pl.d_rg_chunks;
     ra.rank = pl.d_rg_rank;
     ra.runs = pl.d_rg_runs;
-    ra.src = h->d_hbuf;
+    ra.src = hk;
     ra.dst = h->d_hbuf2;
     ra.num_chunks = pl.n_rg_chunks;
     e = lev::launch_regroup(h->cfg.history, ra, h->stream);
   }
   if (e == hipSuccess && mid2) e = hipEventRecord(mid2, h->stream);
-  if (e == hipSuccess)
-    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1,
-                         step_args(h, dt, pl), h->stream);
+  if (e == hipSuccess) {
+    StepArgs a = step_args(h, dt, pl);
+    if (pl.kind == LEV_SCHEDULE_BINNED) a.hbuf = hk;
+    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1, a,
+                         h->stream);
+  }
   return e;
 }
 
 // Enqueue one step on the handle's stream (plus the feature capture while
-// leviathan_step_n_features runs).
-bool enqueue_step(LeviathanHandle* h, float dt) {
-  LEV_HIP(enqueue_main(h, dt));
+// leviathan_step_n_features runs): step k of a block of K.
+bool enqueue_step(LeviathanHandle* h, float dt, int k = 0, int K = 1) {
+  LEV_HIP(enqueue_main(h, dt, k, K));
   LEV_HIP(lev::launch_finalize(fin_args(h, dt, h->plans[h->active]), h->stream));
   if (h->capturing)
     LEV_HIP(lev::launch_capture(h->d_theta, h->d_feat_nodes, h->feat_n, h->d_X, h->feat_max_rows, h->d_step,
                                 h->d_feat_base, h->stream));
+  return true;
+}
+
+// Enqueue n steps: whole blocks of block_steps(h) steps, then single steps.
+bool enqueue_steps(LeviathanHandle* h, float dt, int n) {
+  const int K = block_steps(h);
+  int t = 0;
+  for (; n - t >= K; t += K)
+    for (int k = 0; k < K; ++k)
+      if (!enqueue_step(h, dt, k, K)) return false;
+  for (; t < n; ++t)
+    if (!enqueue_step(h, dt)) return false;
   return true;
 }
 
@@ -370,7 +424,7 @@ bool get_graph(LeviathanHandle* h, float dt, int chunk, hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   LEV_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   bool ok = true;
-  for (int c = 0; c < chunk && ok; ++c) ok = enqueue_step(h, dt);
+  ok = enqueue_steps(h, dt, chunk);
   hipError_t e = hipStreamEndCapture(h->stream, &graph);
   if (!ok) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -405,8 +459,7 @@ bool run_segment(LeviathanHandle* h, float dt, int n, float* r_out) {
       done += chunk;
     }
   }
-  for (; done < n; ++done)
-    if (!enqueue_step(h, dt)) return false;
+  if (done < n && !enqueue_steps(h, dt, n - done)) return false;
   const int64_t first = h->step;
   h->step += n;
   if (r_out) {
@@ -609,6 +662,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   std::vector<uint16_t> bin_rank, st_rank[3], st_lpos[3];
   lev::HostScatterRuns bin_runs, st_runs[3];
   lev::HostRegroup st_rg[3];
+  std::vector<int32_t> early_src, early_cnt;
+  std::vector<uint8_t> early_delay;
+  std::vector<uint32_t> early_csr, bin_early_pos, st_early_pos[3];
   const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
   const bool want_staged = cfg.schedule == LEV_SCHEDULE_STAGED || cfg.schedule == LEV_SCHEDULE_AUTO;
   if (want_binned || want_staged) {
@@ -618,6 +674,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     }
     h->n_sources = (int32_t)hsc.sources.size();
     if (cfg.layout == LEV_LAYOUT_SLOT_MAJOR) lev::order_scatter_by_delay(hsc);
+    lev::early_edges(N, row_ptr, col_idx, delays, lev::kMaxScatterSteps - 1, early_src, early_delay, early_csr,
+                     early_cnt);
+    for (int k = 0; k < lev::kMaxScatterSteps; ++k) h->early_cnt[k] = early_cnt[k];
   }
   if (want_binned) {
     if (!lev::bucket_positions(N, row_ptr, col_idx, cfg.bucket_edges, bin_hpos, h->n_buckets, perr) ||
@@ -627,6 +686,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     }
     lev::sort_scatter_writes(hsc, bin_pos, bin_rank);
     lev::encode_scatter_runs(hsc, bin_pos, bin_runs);
+    bin_early_pos.resize(early_csr.size());
+    for (size_t i = 0; i < early_csr.size(); ++i) bin_early_pos[i] = bin_hpos[early_csr[i]];
     for (int q = 0; q < 3; ++q) {
       DevPlan& pb = h->plans[kPlanBinned + q];
       pb.built = true;
@@ -651,6 +712,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       }
       lev::sort_scatter_writes(hsc, st_pos[q], st_rank[q]);
       lev::encode_scatter_runs(hsc, st_pos[q], st_runs[q]);
+      st_early_pos[q].resize(early_csr.size());
+      for (size_t i = 0; i < early_csr.size(); ++i) st_early_pos[q][i] = bhpos[early_csr[i]];
       DevPlan& ps = h->plans[kPlanStaged + q];
       ps.built = true;
       ps.kind = LEV_SCHEDULE_STAGED;
@@ -733,10 +796,17 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     if (ok && cfg.layout == LEV_LAYOUT_SLOT_MAJOR)
       ok = dalloc(h, &h->d_dsrc, E) &&
            (E == 0 || hip_ok(hipMemcpy(h->d_dsrc, hsc.dsrc.data(), sizeof(uint16_t) * E, H2D), "copy edge sources"));
+    const size_t ne = early_src.size();
+    ok = ok && dalloc(h, &h->d_early_src, ne) && dalloc(h, &h->d_early_delay, ne) &&
+         (ne == 0 || (hip_ok(hipMemcpy(h->d_early_src, early_src.data(), sizeof(int32_t) * ne, H2D), "copy early") &&
+                      hip_ok(hipMemcpy(h->d_early_delay, early_delay.data(), ne, H2D), "copy early")));
     if (!ok) return fail();
     if (want_binned) {
       if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E) ||
-          !upload_runs(h, bin_runs, &h->d_run_idx, &h->d_run_ptr, &h->d_run_base))
+          !upload_runs(h, bin_runs, &h->d_run_idx, &h->d_run_ptr, &h->d_run_base) ||
+          !dalloc(h, &h->d_early_pos, ne))
+        return fail();
+      if (ne && !hip_ok(hipMemcpy(h->d_early_pos, bin_early_pos.data(), sizeof(uint32_t) * ne, H2D), "copy early"))
         return fail();
       if (E && !(hip_ok(hipMemcpy(h->d_pos, bin_pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
                  hip_ok(hipMemcpy(h->d_hpos, bin_hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots") &&
@@ -751,6 +821,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         pb.d_run_idx = h->d_run_idx;
         pb.d_run_ptr = h->d_run_ptr;
         pb.d_run_base = h->d_run_base;
+        pb.d_early_pos = h->d_early_pos;
       }
     }
     if (want_staged) {
@@ -764,7 +835,10 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E) ||
             !upload_runs(h, st_runs[q], &ps.d_run_idx, &ps.d_run_ptr, &ps.d_run_base) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
-            !dalloc(h, &ps.d_rg_runs, rg.runs.size()))
+            !dalloc(h, &ps.d_rg_runs, rg.runs.size()) || !dalloc(h, &ps.d_early_pos, ne))
+          return fail();
+        if (ne && !hip_ok(hipMemcpy(ps.d_early_pos, st_early_pos[q].data(), sizeof(uint32_t) * ne, H2D),
+                          "copy early"))
           return fail();
         if (E && !(hip_ok(hipMemcpy(ps.d_pos, st_pos[q].data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
                    hip_ok(hipMemcpy(ps.d_rank, st_rank[q].data(), sizeof(uint16_t) * E, H2D), "copy ranks") &&
@@ -1237,17 +1311,23 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
     return -1;
   }
   const bool binned = h->plans[h->active].kind != LEV_SCHEDULE_DIRECT;
-  // per step: [start, after scatter, after regroup, after the row pass, after finalize]
+  const bool staged = h->plans[h->active].kind == LEV_SCHEDULE_STAGED;
+  // per step: [start, after scatter, after regroup (staged only), after the row pass, after finalize]
   constexpr int K = 5;
   std::vector<hipEvent_t> ev((size_t)K * n, nullptr);
   bool ok = true;
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) ok = false;
   const FinalizeArgs f = fin_args(h, dt, h->plans[h->active]);
+  // whole blocks of the multi-step scatter, then single steps; the scatter
+  // phase of a step is the block's scatter pass (its first step) or the
+  // gather of its early edges
+  const int B = block_steps(h), full = n / B * B;
   for (int t = 0; t < n && ok; ++t) {
     hipEvent_t* e = &ev[(size_t)K * t];
     ok = hipEventRecord(e[0], h->stream) == hipSuccess &&
-         enqueue_main(h, dt, binned ? e[1] : nullptr, binned ? e[2] : nullptr) == hipSuccess &&
+         enqueue_main(h, dt, t < full ? t % B : 0, t < full ? B : 1, binned ? e[1] : nullptr,
+                      staged ? e[2] : nullptr) == hipSuccess &&
          hipEventRecord(e[3], h->stream) == hipSuccess && lev::launch_finalize(f, h->stream) == hipSuccess &&
          hipEventRecord(e[4], h->stream) == hipSuccess;
   }
@@ -1257,10 +1337,13 @@ int leviathan_profile_steps(LeviathanHandle* h, float dt, int n, double* out) {
     for (int t = 0; t < n; ++t) {
       hipEvent_t* e = &ev[(size_t)K * t];
       float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
-      if (binned) {
+      if (staged) {
         (void)hipEventElapsedTime(&c, e[0], e[1]);
         (void)hipEventElapsedTime(&d, e[1], e[2]);
         (void)hipEventElapsedTime(&a, e[2], e[3]);
+      } else if (binned) {
+        (void)hipEventElapsedTime(&c, e[0], e[1]);
+        (void)hipEventElapsedTime(&a, e[1], e[3]);
       } else {
         (void)hipEventElapsedTime(&a, e[0], e[3]);
       }
@@ -1314,6 +1397,95 @@ int leviathan_get_schedule(LeviathanHandle* h, int* schedule, int* ept) {
   return 0;
 }
 
+namespace {
+
+// Copy of everything a step changes, for trial steps that are rolled back.
+struct StateSnapshot {
+  struct Buf {
+    void* dev;
+    size_t bytes;
+    void* save;
+  };
+  std::vector<Buf> bufs;
+  int64_t step = 0;
+  PhiHandle* phi = nullptr;
+  LeviathanHandle* h = nullptr;
+
+  // Takes the snapshot and detaches the hub statistics (trial steps record nothing).
+  bool take(LeviathanHandle* hh) {
+    h = hh;
+    const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+    const size_t ring_bytes = (h->cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)h->N * h->ring_pitch
+                                                                        : (size_t)h->n_slots * h->ring_pitch) *
+                              hist_bytes;
+    bufs = {{h->d_w, sizeof(float) * (size_t)h->E, nullptr},
+            {h->d_theta, sizeof(float) * (size_t)h->N, nullptr},
+            {h->d_theta_hat, sizeof(float) * (size_t)h->N, nullptr},
+            {h->d_omega_hat, sizeof(float) * (size_t)h->N, nullptr},
+            {h->d_ring, ring_bytes, nullptr},
+            {h->d_ctrl, sizeof(NetCtrl) * (size_t)h->n_nets, nullptr},
+            {h->d_step, sizeof(int64_t), nullptr}};
+    step = h->step;
+    phi = h->phi;
+    h->phi = nullptr;
+    bool ok = hipStreamSynchronize(h->stream) == hipSuccess;
+    for (auto& b : bufs) {
+      if (!ok) break;
+      ok = hipMalloc(&b.save, std::max<size_t>(b.bytes, 1)) == hipSuccess &&
+           hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+    }
+    return ok;
+  }
+  bool restore() {
+    bool ok = true;
+    for (auto& b : bufs)
+      ok = ok && hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+    return ok;
+  }
+  // Restores the state, the step count and the hub statistics, and frees the copy.
+  bool finish() {
+    bool ok = restore() && hipStreamSynchronize(h->stream) == hipSuccess;
+    for (auto& b : bufs)
+      if (b.save) (void)hipFree(b.save);
+    bufs.clear();
+    h->step = step;
+    h->phi = phi;
+    return ok;
+  }
+};
+
+// ms per step of `steps` steps (whole blocks of the multi-step scatter) from the snapshot, after one warm-up block
+bool time_trial(LeviathanHandle* h, StateSnapshot& snap, float dt, int steps, hipEvent_t e0, hipEvent_t e1,
+                double* per) {
+  const int n = std::max(steps / block_steps(h), 1) * block_steps(h);
+  bool ok = snap.restore() && enqueue_steps(h, dt, block_steps(h)) && hipEventRecord(e0, h->stream) == hipSuccess &&
+            enqueue_steps(h, dt, n) && hipEventRecord(e1, h->stream) == hipSuccess &&
+            hipEventSynchronize(e1) == hipSuccess;
+  float ms = 0.f;
+  if (ok) (void)hipEventElapsedTime(&ms, e0, e1);
+  *per = ms / n;
+  return ok;
+}
+
+bool resize_hbuf(LeviathanHandle* h, int steps) {
+  if (steps <= h->hbuf_steps) return true;
+  const size_t one = std::max<size_t>((size_t)h->E * (h->cfg.history == LEV_HISTORY_F32 ? 4 : 2), 1);
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, one * (size_t)steps);
+  if (e != hipSuccess) {
+    set_err("hipMalloc(%zu bytes) for the values of %d steps failed: %s", one * (size_t)steps, steps,
+            hipGetErrorString(e));
+    return false;
+  }
+  (void)hipFree(h->d_hbuf);
+  h->vram += one * (size_t)(steps - h->hbuf_steps);
+  h->d_hbuf = p;
+  h->hbuf_steps = steps;
+  return true;
+}
+
+}  // namespace
+
 int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) {
   if (!check_handle(h)) return -1;
   if (steps <= 0) {
@@ -1324,32 +1496,9 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
     set_err("autotune runs undriven steps; switch the drive off first");
     return -1;
   }
-  // snapshot of everything a step changes
-  const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  const size_t ring_bytes = (h->cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)h->N * h->ring_pitch
-                                                                      : (size_t)h->n_slots * h->ring_pitch) *
-                            hist_bytes;
-  struct Buf {
-    void* dev;
-    size_t bytes;
-    void* save;
-  } bufs[] = {{h->d_w, sizeof(float) * (size_t)h->E, nullptr},
-              {h->d_theta, sizeof(float) * (size_t)h->N, nullptr},
-              {h->d_theta_hat, sizeof(float) * (size_t)h->N, nullptr},
-              {h->d_omega_hat, sizeof(float) * (size_t)h->N, nullptr},
-              {h->d_ring, ring_bytes, nullptr},
-              {h->d_ctrl, sizeof(NetCtrl) * (size_t)h->n_nets, nullptr},
-              {h->d_step, sizeof(int64_t), nullptr}};
-  bool ok = hipStreamSynchronize(h->stream) == hipSuccess;
-  for (auto& b : bufs) {
-    if (!ok) break;
-    ok = hipMalloc(&b.save, std::max<size_t>(b.bytes, 1)) == hipSuccess &&
-         hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
-  }
-  const int64_t step0 = h->step;
+  StateSnapshot snap;
+  bool ok = snap.take(h);
   const int active0 = h->active;
-  PhiHandle* const phi0 = h->phi;  // trial steps are rolled back: they record no hub transitions
-  h->phi = nullptr;
   double best = 1e300;
   int best_idx = active0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1358,15 +1507,8 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
     if (ms_out) ms_out[q] = std::numeric_limits<double>::quiet_NaN();
     if (!ok || !h->plans[q].built) continue;
     h->active = q;
-    // restore the snapshot, warm up one step, time `steps` steps
-    for (auto& b : bufs)
-      ok = ok && hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
-    ok = ok && enqueue_step(h, dt) && hipEventRecord(e0, h->stream) == hipSuccess;
-    for (int t = 0; t < steps && ok; ++t) ok = enqueue_step(h, dt);
-    ok = ok && hipEventRecord(e1, h->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-    float ms = 0.f;
-    if (ok) (void)hipEventElapsedTime(&ms, e0, e1);
-    const double per = ms / steps;
+    double per = 0.0;
+    ok = time_trial(h, snap, dt, steps, e0, e1, &per);
     if (ms_out) ms_out[q] = per;
     if (ok && per < best) {
       best = per;
@@ -1374,19 +1516,87 @@ int leviathan_autotune(LeviathanHandle* h, float dt, int steps, double* ms_out) 
     }
   }
   // restore the state the call started from and keep the fastest plan
-  for (auto& b : bufs)
-    ok = ok && hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
-  ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
-  for (auto& b : bufs)
-    if (b.save) (void)hipFree(b.save);
+  ok = snap.finish() && ok;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  h->step = step0;
-  h->phi = phi0;
   if (best_idx != active0) drop_graphs(h);
   h->active = ok ? best_idx : active0;
   if (!ok) {
     set_err("autotune: a HIP call failed");
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_set_scatter_steps(LeviathanHandle* h, int steps) {
+  if (!check_handle(h)) return -1;
+  if (steps != 1 && steps != 2 && steps != lev::kMaxScatterSteps) {
+    set_err("scatter steps must be 1, 2 or %d (got %d)", lev::kMaxScatterSteps, steps);
+    return -1;
+  }
+  if (!h->d_hbuf) {
+    set_err("no two-phase schedule was built for this handle");
+    return -1;
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_err("hipStreamSynchronize failed");
+    return -1;
+  }
+  if (!resize_hbuf(h, steps)) return -1;
+  if (steps != h->scatter_steps) drop_graphs(h);  // captured graphs hold the old block structure
+  h->scatter_steps = steps;
+  return 0;
+}
+
+int leviathan_get_scatter_steps(LeviathanHandle* h) {
+  if (!check_handle(h)) return -1;
+  return h->scatter_steps;
+}
+
+int leviathan_autotune_scatter_steps(LeviathanHandle* h, float dt, int steps, double* ms_out) {
+  if (!check_handle(h)) return -1;
+  if (steps <= 0) {
+    set_err("steps must be positive");
+    return -1;
+  }
+  if (h->drive_on) {
+    set_err("autotune runs undriven steps; switch the drive off first");
+    return -1;
+  }
+  const int cand[3] = {1, 2, lev::kMaxScatterSteps};
+  for (int i = 0; i < 3; ++i)
+    if (ms_out) ms_out[i] = std::numeric_limits<double>::quiet_NaN();
+  if (h->plans[h->active].kind == LEV_SCHEDULE_DIRECT) return 0;  // the direct schedule has no scatter
+  StateSnapshot snap;
+  bool ok = snap.take(h);
+  const int k0 = h->scatter_steps;
+  double best = 1e300;
+  int best_k = k0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (int i = 0; i < 3 && ok; ++i) {
+    if (!resize_hbuf(h, cand[i])) {
+      snap.finish();
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      return -1;
+    }
+    h->scatter_steps = cand[i];
+    double per = 0.0;
+    ok = time_trial(h, snap, dt, steps, e0, e1, &per);
+    if (ms_out) ms_out[i] = per;
+    if (ok && per < best) {
+      best = per;
+      best_k = cand[i];
+    }
+  }
+  ok = snap.finish() && ok;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  h->scatter_steps = ok ? best_k : k0;
+  if (h->scatter_steps != k0) drop_graphs(h);
+  if (!ok) {
+    set_err("autotune_scatter_steps: a HIP call failed");
     return -1;
   }
   return 0;

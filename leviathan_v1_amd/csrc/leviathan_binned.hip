@@ -35,12 +35,32 @@
 
 namespace lev {
 
-template <class H, int LAYOUT>
-__global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
-  constexpr int EPT = kScatterEpt;
-  constexpr int CAP = kBlock * EPT;
+// lanes of a scatter workgroup gathering K steps per pass (measured at BA
+// N = 10^7, u16, K = 4: 1.82 ms per pass with 256 lanes and 16 edges per lane
+// at 164 VGPRs, 1.92 ms with 512 lanes and 8 edges per lane at 92 VGPRs)
+constexpr int scatter_threads(int) { return kBlock; }
+
+// slot of theta(t + k - d) on a ring of S slots, cur = t % S, 0 <= d <= S - 2
+// and 0 <= k < kMaxScatterSteps
+__device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d, int32_t S) {
+  int32_t slot = cur + k - d;
+  slot += (slot < 0) ? S : 0;
+  slot -= (slot >= S) ? S : 0;
+  return slot;
+}
+
+// K = number of consecutive steps whose values one pass gathers (1, 2 or 4):
+// step t + k's values go to hbuf + k * hbuf_stride.  Values of edges with
+// d < k are read from slots this pass must not trust (not yet written for
+// step t + k); early_kernel overwrites them before that step's row pass.
+template <class H, int LAYOUT, int K>
+__global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs a) {
+  // a source block holds up to CAP out-edges, EPT per lane
+  constexpr int T = scatter_threads(K);
+  constexpr int CAP = kBlock * kScatterEpt;
+  constexpr int EPT = CAP / T;
   __shared__ uint8_t s_esrc[CAP];
-  __shared__ H s_val[CAP];                  // the block's values in slot order
+  __shared__ H s_val[K > 1 ? 2 : 1][CAP + 1];  // the block's values in slot order (two buffers when K > 1)
   __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
@@ -81,18 +101,18 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // the block's runs of consecutive slots: slot-order index k of run r goes
     // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
     const int32_t rb = a.run_ptr[b], nr = min(a.run_ptr[b + 1] - rb, kScatterRuns);
-    int32_t tb[kScatterRuns / kBlock];
+    int32_t tb[kScatterRuns / T];
 #pragma unroll
-    for (int j = 0; j < kScatterRuns / kBlock; ++j) {
-      const int32_t r = j * kBlock + tid;
+    for (int j = 0; j < kScatterRuns / T; ++j) {
+      const int32_t r = j * T + tid;
       tb[j] = a.run_base[r < nr ? rb + r : 0];
     }
     // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
     int32_t dd[EPT];
-    int32_t rk[EPT];
+    int32_t rk[EPT];  // rank; when K > 1 also the run index << 16
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
+      const int32_t le = k * T + tid;
       const int32_t c = le < nnz ? c0 + le : 0;
       if constexpr (LAYOUT == 0) {
         dd[k] = __builtin_nontemporal_load(a.delay + c);
@@ -101,82 +121,121 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       }
       rk[k] = __builtin_nontemporal_load(a.rank + c);
     }
+    int32_t ri[K > 1 ? 1 : EPT];
     // node-major: edges in source order, so the lanes of one source share its
     // history line (read with the streaming hint: only this block reuses it,
-    // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges in
-    // (delay, source) order, so the lanes of one load read one slot row for
-    // consecutive sources.
-    H v[EPT];
+    // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges
+    // in (delay, source) order, so the lanes of one load read one slot row for
+    // consecutive sources; for q > 0 those rows were read for q - 1 by the
+    // neighbouring delay and are in cache.
+    auto load_values = [&](H (&v)[EPT], int q) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      int32_t slot = cur - (dd[k] & 255);
-      slot += (slot < 0) ? S : 0;
-      int32_t node;
-      if constexpr (LAYOUT == 0) {
-        node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-      } else {
-        node = s0 + (dd[k] >> 8);
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;
+        const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
+        int32_t node;
+        if constexpr (LAYOUT == 0) {
+          node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+        } else {
+          node = s0 + (dd[k] >> 8);
+        }
+        const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
+        v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
       }
-      const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
-      v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
-    }
-    // place in slot order
+    };
+    H v[EPT], vn[EPT];
+    load_values(v, 0);
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      if (le < nnz) s_val[rk[k]] = v[k];
-    }
+    for (int q = 0; q < K; ++q) {
+      // the next step's values are in flight across this step's barrier
+      if (q + 1 < K) load_values(vn, q + 1);
+      H* sv = s_val[q & 1];
+      // place in slot order; a lane past the block's edges writes the spare
+      // slot CAP, so that no load is sunk into a branch and waited for alone
 #pragma unroll
-    for (int j = 0; j < kScatterRuns / kBlock; ++j) {
-      const int32_t r = j * kBlock + tid;
-      if (r < nr) s_base[r] = tb[j];
-    }
-    // each slot-order index's run, loaded once the delays and ranks are no
-    // longer live (fewer registers)
-    int32_t ri[EPT];
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;
+        sv[le < nnz ? (rk[k] & 0xffff) : CAP] = v[k];
+      }
+      if (q == 0) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      ri[k] = __builtin_nontemporal_load(a.run_idx + (le < nnz ? c0 + le : 0));
-    }
-    __syncthreads();
-    // write in slot order: the values bound for one bucket leave as one contiguous run
+        for (int j = 0; j < kScatterRuns / T; ++j) {
+          const int32_t r = j * T + tid;
+          if (r < nr) s_base[r] = tb[j];
+        }
+        // each slot-order index's run, loaded once the delays are no longer
+        // live (fewer registers); with several steps it rides in rk's high half
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
-        const int32_t base = ri[k] < kScatterRuns ? s_base[ri[k]] : a.run_base[rb + ri[k]];
-        hbuf[base + le] = s_val[le];
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;
+          const int32_t r = __builtin_nontemporal_load(a.run_idx + (le < nnz ? c0 + le : 0));
+          if constexpr (K > 1)
+            rk[k] |= r << 16;
+          else
+            ri[k] = r;
+        }
+      }
+      __syncthreads();
+      // write in slot order: the values bound for one bucket leave as one contiguous run
+      H* out = hbuf + (int64_t)q * a.hbuf_stride;
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;
+        if (le < nnz) {
+          const int32_t r = K > 1 ? (int32_t)((uint32_t)rk[k] >> 16) : ri[K > 1 ? 0 : k];
+          const int32_t base = r < kScatterRuns ? s_base[r] : a.run_base[rb + r];
+          out[base + le] = sv[le];
+        }
+      }
+      if (q + 1 < K) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) v[k] = vn[k];
       }
     }
   } else {
     // one source with more than CAP out-edges: the whole workgroup strides over
     // them, four edges per lane at a time with unconditional loads
-    for (int32_t base = 0; base < nnz; base += 4 * kBlock) {
+    for (int32_t base = 0; base < nnz; base += 4 * T) {
       int32_t d[4];
       uint32_t p[4];
-      H v[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int32_t le = base + k * kBlock + tid;
+        const int32_t le = base + k * T + tid;
         const int32_t c = c0 + (le < nnz ? le : 0);
         d[k] = __builtin_nontemporal_load(a.delay + c);
         p[k] = __builtin_nontemporal_load(a.pos + c);
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int32_t slot = cur - d[k];
-        slot += (slot < 0) ? S : 0;
-        v[k] = ring[ring_index<LAYOUT>(pitch, s0, slot)];  // a hub's line: reused, cached
-      }
+      for (int q = 0; q < K; ++q) {
+        H v[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int32_t le = base + k * kBlock + tid;
-        if (le < nnz) hbuf[p[k]] = v[k];
+        for (int k = 0; k < 4; ++k)
+          v[k] = ring[ring_index<LAYOUT>(pitch, s0, ahead_slot(cur, q, d[k], S))];  // a hub's line: reused, cached
+        H* out = hbuf + (int64_t)q * a.hbuf_stride;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int32_t le = base + k * T + tid;
+          if (le < nnz) out[p[k]] = v[k];
+        }
       }
     }
   }
+}
+
+// The values of the edges with delay < k for step t + k of a multi-step
+// scatter, read once step t + k - 1 has written theta(t + k - 1): one lane per
+// edge, in (delay, source) order so that the lanes of a load read one slot row
+// for ascending sources.
+template <class H, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void early_kernel(EarlyArgs a) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.count) return;
+  const int32_t S = a.n_slots;
+  const int32_t cur = (int32_t)(*a.step % S);
+  int32_t slot = cur - (int32_t)a.delay[i];
+  slot += (slot < 0) ? S : 0;
+  const H* ring = reinterpret_cast<const H*>(a.ring);
+  reinterpret_cast<H*>(a.hbuf)[a.pos[i]] = ring[ring_index<LAYOUT>(a.ring_pitch, a.src[i], slot)];
 }
 
 // Regroup pass of the staged schedule: kRegroupChunksPerWg consecutive chunks
@@ -282,19 +341,46 @@ hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s) {
-  if (a.num_blocks <= 0) return hipSuccess;
-  const dim3 grid(a.num_blocks), block(kBlock);
+template <int K>
+static void scatter_launch(int history, int layout, const ScatterArgs& a, hipStream_t s) {
+  const dim3 grid(a.num_blocks), block(scatter_threads(K));
   if (history == 0) {
     if (layout == 0)
-      scatter_kernel<float, 0><<<grid, block, 0, s>>>(a);
+      scatter_kernel<float, 0, K><<<grid, block, 0, s>>>(a);
     else
-      scatter_kernel<float, 1><<<grid, block, 0, s>>>(a);
+      scatter_kernel<float, 1, K><<<grid, block, 0, s>>>(a);
   } else {
     if (layout == 0)
-      scatter_kernel<uint16_t, 0><<<grid, block, 0, s>>>(a);
+      scatter_kernel<uint16_t, 0, K><<<grid, block, 0, s>>>(a);
     else
-      scatter_kernel<uint16_t, 1><<<grid, block, 0, s>>>(a);
+      scatter_kernel<uint16_t, 1, K><<<grid, block, 0, s>>>(a);
+  }
+}
+
+hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s) {
+  if (a.num_blocks <= 0) return hipSuccess;
+  switch (a.steps) {
+    case 1: scatter_launch<1>(history, layout, a, s); break;
+    case 2: scatter_launch<2>(history, layout, a, s); break;
+    case kMaxScatterSteps: scatter_launch<kMaxScatterSteps>(history, layout, a, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_early(int history, int layout, const EarlyArgs& a, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const dim3 grid((a.count + kBlock - 1) / kBlock), block(kBlock);
+  if (history == 0) {
+    if (layout == 0)
+      early_kernel<float, 0><<<grid, block, 0, s>>>(a);
+    else
+      early_kernel<float, 1><<<grid, block, 0, s>>>(a);
+  } else {
+    if (layout == 0)
+      early_kernel<uint16_t, 0><<<grid, block, 0, s>>>(a);
+    else
+      early_kernel<uint16_t, 1><<<grid, block, 0, s>>>(a);
   }
   return hipGetLastError();
 }

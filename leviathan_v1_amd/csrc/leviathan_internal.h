@@ -128,7 +128,27 @@ struct ScatterArgs {
   int32_t num_blocks;
   int32_t xcd_remap;
   const int4* blocks;       // pack_item layout: {first source, first out-edge, end out-edge, sources << 1 | long}
-  void* hbuf;               // [E] delayed phases, bucketed
+  void* hbuf;               // [steps][hbuf_stride] delayed phases, bucketed
+  int64_t hbuf_stride;      // elements between the value arrays of consecutive steps
+  int32_t steps;            // 1, 2 or kMaxScatterSteps steps gathered in one pass
+  const int64_t* step;
+};
+
+// One scatter pass can gather the values of up to kMaxScatterSteps consecutive
+// steps t..t+K-1: step t+k reads theta(t+k-d), which exists before step t's
+// row pass whenever d >= k.  The remaining edges (d < k) are gathered by
+// early_kernel just before step t+k's row pass, from a list of the edges with
+// delay < kMaxScatterSteps - 1 sorted by (delay, source).
+constexpr int kMaxScatterSteps = 4;
+struct EarlyArgs {
+  const int32_t* src;       // [count] source node
+  const uint32_t* pos;      // [count] value slot in the step's bucketed array
+  const uint8_t* delay;     // [count] delay, ascending
+  int32_t count;            // the records with delay < k
+  const void* ring;
+  int64_t ring_pitch;
+  int32_t n_slots;
+  void* hbuf;               // the bucketed values of the step being prepared
   const int64_t* step;
 };
 
@@ -145,6 +165,7 @@ hipError_t launch_read_slot(int history, int layout, const void* ring, int64_t p
                             float* out, int32_t n, hipStream_t s);
 // (leviathan_binned.hip)
 hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s);
+hipError_t launch_early(int history, int layout, const EarlyArgs& a, hipStream_t s);
 // (reservoir.hip)
 hipError_t launch_capture(const float* theta, const int32_t* nodes, int32_t n_nodes, float* X, int64_t max_rows,
                           const int64_t* step, const int64_t* base, hipStream_t s);

@@ -399,4 +399,26 @@ bool csc_positions(const HostScatter& sc, const std::vector<uint32_t>& hpos, std
   return true;
 }
 
+void early_edges(int N, const int* row_ptr, const int* col_idx, const uint8_t* delays, int below,
+                 std::vector<int32_t>& src, std::vector<uint8_t>& delay, std::vector<uint32_t>& csr,
+                 std::vector<int32_t>& cnt) {
+  std::vector<std::vector<uint64_t>> keys(std::max(below, 0));
+  for (int i = 0; i < N; ++i)
+    for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+      if (delays[e] < below) keys[delays[e]].push_back((uint64_t)(uint32_t)col_idx[e] << 32 | (uint32_t)e);
+  cnt.assign((size_t)std::max(below, 0) + 1, 0);
+  src.clear();
+  delay.clear();
+  csr.clear();
+  for (int d = 0; d < below; ++d) {
+    std::sort(keys[d].begin(), keys[d].end());
+    for (uint64_t k : keys[d]) {
+      src.push_back((int32_t)(k >> 32));
+      delay.push_back((uint8_t)d);
+      csr.push_back((uint32_t)k);
+    }
+    cnt[d + 1] = (int32_t)src.size();
+  }
+}
+
 }  // namespace lev

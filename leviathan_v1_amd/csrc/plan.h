@@ -110,4 +110,11 @@ void encode_scatter_runs(const HostScatter& sc, const std::vector<uint32_t>& pos
 // Must run before the positions are derived from csc_edge.
 void order_scatter_by_delay(HostScatter& sc);
 
+// Multi-step scatter: the CSR edges with delay < below, sorted by (delay,
+// source, CSR index).  src/delay/csr hold each record's source, delay and CSR
+// edge; cnt[d] (d = 0..below) = the number of records with delay < d.
+void early_edges(int N, const int* row_ptr, const int* col_idx, const uint8_t* delays, int below,
+                 std::vector<int32_t>& src, std::vector<uint8_t>& delay, std::vector<uint32_t>& csr,
+                 std::vector<int32_t>& cnt);
+
 }  // namespace lev

@@ -388,6 +388,27 @@ class LeviathanEngine:
             _lib.check("autotune")
         return {n: float(v) for n, v in zip(_lib.PLAN_NAMES, out) if v == v}
 
+    @property
+    def scatter_steps(self) -> int:
+        """Consecutive steps of step_n whose delayed phases one scatter pass gathers (two-phase schedules)."""
+        k = _lib.lib().leviathan_get_scatter_steps(self._h)
+        if k < 0:
+            _lib.check("get_scatter_steps")
+        return k
+
+    def set_scatter_steps(self, steps: int) -> None:
+        """1, 2 or 4 steps per scatter pass; results are bit-identical for every choice."""
+        if _lib.lib().leviathan_set_scatter_steps(self._h, int(steps)) != 0:
+            _lib.check("set_scatter_steps")
+
+    def autotune_scatter_steps(self, dt: float, steps: int = 12) -> dict:
+        """Times 1, 2 and 4 steps per scatter pass on the active schedule from the current state (restored)
+        and keeps the fastest.  Returns ms per step of each (empty for the direct schedule)."""
+        out = np.zeros(3, dtype=np.float64)
+        if _lib.lib().leviathan_autotune_scatter_steps(self._h, float(dt), int(steps), _p(out)) != 0:
+            _lib.check("autotune_scatter_steps")
+        return {k: float(v) for k, v in zip((1, 2, 4), out) if v == v}
+
     def close(self) -> None:
         if self._h is not None:
             _lib.lib().leviathan_free(self._h)

@@ -21,7 +21,8 @@ from collections import defaultdict
 
 
 def _short(name: str) -> str:
-    return name.replace("(lev::StepArgs)", "").replace("(lev::ScatterArgs)", "").replace("void ", "").strip()
+    return (name.replace("(lev::StepArgs)", "").replace("(lev::ScatterArgs)", "").replace("(lev::EarlyArgs)", "")
+            .replace("void ", "").strip())
 
 
 def stats(path, cmd, top=8):
@@ -66,7 +67,7 @@ def traffic(fetch_csv, write_csv, copy_bytes, key, cmd_fetch, cmd_write, json_pa
         fv = sum(fa[k]) / len(fa[k]) * 1024 * f_scale
         wv = sum(wa.get(k, [0.0])) / max(len(wa.get(k, [1])), 1) * 1024 * w_scale
         lines.append(f"| {k} | {len(fa[k])} | {fv / 1e9:.3f} | {wv / 1e9:.3f} |")
-        rec[k] = {"fetch_bytes": fv, "write_bytes": wv}
+        rec[k] = {"fetch_bytes": fv, "write_bytes": wv, "dispatches": len(fa[k])}
     db = {}
     if os.path.exists(json_path):
         with open(json_path) as f:
