@@ -71,8 +71,9 @@ struct DevPlan {
   int32_t xcd_split[lev::kNumXcd + 1] = {};
   bool owns = true;            // binned and staged plans share the row blocks of the direct plan with the same ept
   uint32_t* d_pos = nullptr;   // two-phase plans: [E] value slots in the scatter kernel's write order
-  uint16_t* d_rank = nullptr;  // two-phase plans: [E] CSC edge -> index in its source block's write order
-  uint16_t* d_run_idx = nullptr;  // two-phase plans: the scatter's runs of consecutive slots
+  // two-phase plans, [E]: CSC edge -> index in its source block's slot order (low half) and,
+  // per slot-order index, its run of consecutive slots (high half)
+  uint32_t* d_rank_run = nullptr;
   int32_t* d_run_ptr = nullptr;
   int32_t* d_run_base = nullptr;
   uint16_t* d_lpos = nullptr;  // staged plans: [E] CSR edge -> slot within its block's range
@@ -124,8 +125,7 @@ struct LeviathanHandle {
   void* d_hbuf = nullptr;    // two-phase schedules: the scatter kernel's bucketed values
   void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block
   uint32_t* d_hpos = nullptr;
-  uint16_t* d_rank = nullptr;
-  uint16_t* d_run_idx = nullptr;  // binned plans' scatter runs
+  uint32_t* d_rank_run = nullptr;  // binned plans' ranks and scatter runs
   int32_t* d_run_ptr = nullptr;
   int32_t* d_run_base = nullptr;
   // multi-step scatter: the edges with delay < kMaxScatterSteps - 1 by (delay, source)
@@ -189,12 +189,21 @@ bool dalloc(LeviathanHandle* h, T** p, size_t count) {
   return true;
 }
 
-bool upload_runs(LeviathanHandle* h, const lev::HostScatterRuns& r, uint16_t** idx, int32_t** ptr, int32_t** base) {
-  if (!dalloc(h, idx, r.run_idx.size()) || !dalloc(h, ptr, r.run_ptr.size()) || !dalloc(h, base, r.run_base.size()))
+// The scatter's per-edge rank and per-slot-order-index run, packed rank | run << 16 (both are
+// indexed by the block's first out-edge + the lane's index), and its run tables.
+bool upload_runs(LeviathanHandle* h, const std::vector<uint16_t>& rank, const lev::HostScatterRuns& r,
+                 uint32_t** rank_run, int32_t** ptr, int32_t** base) {
+  if (rank.size() != r.run_idx.size()) {
+    set_err("scatter ranks and runs differ in length");
     return false;
+  }
+  if (!dalloc(h, rank_run, rank.size()) || !dalloc(h, ptr, r.run_ptr.size()) || !dalloc(h, base, r.run_base.size()))
+    return false;
+  std::vector<uint32_t> packed(rank.size());
+  for (size_t c = 0; c < rank.size(); ++c) packed[c] = (uint32_t)rank[c] | (uint32_t)r.run_idx[c] << 16;
   hipError_t e = hipSuccess;
-  if (!r.run_idx.empty())
-    e = hipMemcpy(*idx, r.run_idx.data(), sizeof(uint16_t) * r.run_idx.size(), hipMemcpyHostToDevice);
+  if (!packed.empty())
+    e = hipMemcpy(*rank_run, packed.data(), sizeof(uint32_t) * packed.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(*ptr, r.run_ptr.data(), sizeof(int32_t) * r.run_ptr.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
@@ -220,14 +229,14 @@ void destroy(LeviathanHandle* h) {
   void* ptrs[] = {h->d_row_ptr, h->d_col,       h->d_delay,   h->d_w,        h->d_theta,   h->d_theta_hat,
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_fin_done, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_dsrc,
-                  h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank,     h->d_run_idx, h->d_run_ptr, h->d_run_base, h->d_sources,  h->d_feat_nodes,
+                  h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank_run, h->d_run_ptr, h->d_run_base, h->d_sources,  h->d_feat_nodes,
                   h->d_early_src, h->d_early_delay, h->d_early_pos,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
     if (pl.owns_pos) {
-      void* pp[] = {pl.d_pos,   pl.d_rank,      pl.d_run_idx,  pl.d_run_ptr, pl.d_run_base,
+      void* pp[] = {pl.d_pos,   pl.d_rank_run,  pl.d_run_ptr, pl.d_run_base,
                     pl.d_lpos,  pl.d_rg_chunks, pl.d_rg_rank,  pl.d_rg_runs, pl.d_early_pos};
       for (void* p : pp)
         if (p) (void)hipFree(p);
@@ -317,8 +326,7 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   sa.delay = h->d_delay_csc;
   sa.dsrc = h->d_dsrc;
   sa.pos = pl.d_pos;
-  sa.rank = pl.d_rank;
-  sa.run_idx = pl.d_run_idx;
+  sa.rank_run = pl.d_rank_run;
   sa.run_ptr = pl.d_run_ptr;
   sa.run_base = pl.d_run_base;
   sa.ring = h->d_ring;
@@ -802,23 +810,21 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
                       hip_ok(hipMemcpy(h->d_early_delay, early_delay.data(), ne, H2D), "copy early")));
     if (!ok) return fail();
     if (want_binned) {
-      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) || !dalloc(h, &h->d_rank, E) ||
-          !upload_runs(h, bin_runs, &h->d_run_idx, &h->d_run_ptr, &h->d_run_base) ||
+      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) ||
+          !upload_runs(h, bin_rank, bin_runs, &h->d_rank_run, &h->d_run_ptr, &h->d_run_base) ||
           !dalloc(h, &h->d_early_pos, ne))
         return fail();
       if (ne && !hip_ok(hipMemcpy(h->d_early_pos, bin_early_pos.data(), sizeof(uint32_t) * ne, H2D), "copy early"))
         return fail();
       if (E && !(hip_ok(hipMemcpy(h->d_pos, bin_pos.data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
-                 hip_ok(hipMemcpy(h->d_hpos, bin_hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots") &&
-                 hip_ok(hipMemcpy(h->d_rank, bin_rank.data(), sizeof(uint16_t) * E, H2D), "copy ranks")))
+                 hip_ok(hipMemcpy(h->d_hpos, bin_hpos.data(), sizeof(uint32_t) * E, H2D), "copy slots")))
         return fail();
       for (int q = 0; q < 3; ++q) {
         DevPlan& pb = h->plans[kPlanBinned + q];
         pb.d_items = h->plans[q].d_items;
         pb.d_ranges = h->plans[q].d_ranges;
         pb.d_pos = h->d_pos;
-        pb.d_rank = h->d_rank;
-        pb.d_run_idx = h->d_run_idx;
+        pb.d_rank_run = h->d_rank_run;
         pb.d_run_ptr = h->d_run_ptr;
         pb.d_run_base = h->d_run_base;
         pb.d_early_pos = h->d_early_pos;
@@ -832,8 +838,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         ps.owns_pos = true;
         const lev::HostRegroup& rg = st_rg[q];
         ps.n_rg_chunks = (int32_t)rg.chunks.size();
-        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_rank, E) || !dalloc(h, &ps.d_lpos, E) ||
-            !upload_runs(h, st_runs[q], &ps.d_run_idx, &ps.d_run_ptr, &ps.d_run_base) ||
+        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_lpos, E) ||
+            !upload_runs(h, st_rank[q], st_runs[q], &ps.d_rank_run, &ps.d_run_ptr, &ps.d_run_base) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
             !dalloc(h, &ps.d_rg_runs, rg.runs.size()) || !dalloc(h, &ps.d_early_pos, ne))
           return fail();
@@ -841,7 +847,6 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
                           "copy early"))
           return fail();
         if (E && !(hip_ok(hipMemcpy(ps.d_pos, st_pos[q].data(), sizeof(uint32_t) * E, H2D), "copy positions") &&
-                   hip_ok(hipMemcpy(ps.d_rank, st_rank[q].data(), sizeof(uint16_t) * E, H2D), "copy ranks") &&
                    hip_ok(hipMemcpy(ps.d_lpos, st_lpos[q].data(), sizeof(uint16_t) * E, H2D), "copy slots") &&
                    hip_ok(hipMemcpy(ps.d_rg_chunks, rg.chunks.data(), sizeof(int4) * rg.chunks.size(), H2D),
                           "copy regroup chunks") &&

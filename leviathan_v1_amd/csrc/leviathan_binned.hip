@@ -24,9 +24,11 @@
 //     contiguous run of row blocks, so the bucket being read (1 MB) stays in
 //     that XCD's 4 MB L2 and is fetched from HBM about once.
 //
-// Per edge the HBM traffic is delay 1 + rank 2 + position 4 + h write 2|4 (phase 1) and
-// position 4 + h read 2|4 + weight read/write 8 (phase 2), plus one pass over
-// the history ring, instead of a cache line per edge.
+// Per edge of a short source block the HBM traffic is source-and-delay code 2
+// (delay 1 on the node-major ring) + rank and run index 4 + h write 2|4
+// (phase 1) and position 4 + h read 2|4 + weight read/write 8 (phase 2), plus
+// one pass over the history ring and the block's run bases, instead of a cache
+// line per edge.  Edges of sources beyond one block read delay 1 + position 4.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs
     }
     // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
     int32_t dd[EPT];
-    int32_t rk[EPT];  // rank; when K > 1 also the run index << 16
+    uint32_t rk[EPT];  // rank | run index << 16
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * T + tid;
@@ -119,9 +121,8 @@ __global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs
       } else {
         dd[k] = le < nnz ? (int32_t)__builtin_nontemporal_load(a.dsrc + c) : 0;
       }
-      rk[k] = __builtin_nontemporal_load(a.rank + c);
+      rk[k] = __builtin_nontemporal_load(a.rank_run + c);
     }
-    int32_t ri[K > 1 ? 1 : EPT];
     // node-major: edges in source order, so the lanes of one source share its
     // history line (read with the streaming hint: only this block reuses it,
     // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges
@@ -163,17 +164,6 @@ __global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs
           const int32_t r = j * T + tid;
           if (r < nr) s_base[r] = tb[j];
         }
-        // each slot-order index's run, loaded once the delays are no longer
-        // live (fewer registers); with several steps it rides in rk's high half
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-          const int32_t le = k * T + tid;
-          const int32_t r = __builtin_nontemporal_load(a.run_idx + (le < nnz ? c0 + le : 0));
-          if constexpr (K > 1)
-            rk[k] |= r << 16;
-          else
-            ri[k] = r;
-        }
       }
       __syncthreads();
       // write in slot order: the values bound for one bucket leave as one contiguous run
@@ -182,7 +172,7 @@ __global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
         if (le < nnz) {
-          const int32_t r = K > 1 ? (int32_t)((uint32_t)rk[k] >> 16) : ri[K > 1 ? 0 : k];
+          const int32_t r = (int32_t)(rk[k] >> 16);
           const int32_t base = r < kScatterRuns ? s_base[r] : a.run_base[rb + r];
           out[base + le] = sv[le];
         }

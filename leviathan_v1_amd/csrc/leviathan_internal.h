@@ -117,8 +117,9 @@ struct ScatterArgs {
   const int32_t* csc_ptr;   // [N+1] out-edge offsets per source
   const uint8_t* delay;     // [E] delays in CSC order
   const uint32_t* pos;      // [E] long sources: value slots in CSC order
-  const uint16_t* rank;     // [E] short source blocks: CSC edge -> its index in the block's slot order
-  const uint16_t* run_idx;  // [E] short source blocks: each slot-order index's run within its block
+  // [E] short source blocks: CSC edge -> its index in the block's slot order (low 16 bits) and each
+  // slot-order index's run within its block (high 16 bits)
+  const uint32_t* rank_run;
   const int32_t* run_ptr;   // [num_blocks + 1] each block's runs of consecutive slots
   const int32_t* run_base;  // slot-order index k of run r goes to slot run_base[r] + k
   const uint16_t* dsrc;     // [E] slot-major ring: each edge's source index in its block << 8 | delay
