@@ -83,6 +83,11 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
             else:
                 eng.set_scatter_steps(int(scatter_steps))
             log(f"{history}: scatter steps {k_ms} -> {eng.scatter_steps}")
+            if schedule == "auto" and eng.scatter_steps > 1:
+                # the plans again, the two-phase ones at that many steps per pass
+                tuned = eng.autotune(wl.dt, steps=8)
+                chosen = eng.get_schedule()
+                log(f"{history}: autotune at {eng.scatter_steps} steps per pass {tuned} -> {chosen}")
         eng.step_n(wl.dt, warmup)
         t0 = time.perf_counter()
         r = eng.step_n(wl.dt, steps)  # returns after the stream is synchronised
@@ -169,8 +174,10 @@ def load_traffic(workload, history, layout, schedule, scatter_steps=None):
         return None
     byt = lambda v: v["fetch_bytes"] + v["write_bytes"]
     kern = rec["kernels"]
-    total = sum(byt(v) for k, v in kern.items() if "step_kernel" in k or "regroup_kernel" in k)
     K = scatter_steps if scatter_steps and scatter_steps > 1 else 1
+    total = sum(byt(v) for k, v in kern.items() if "step_kernel" in k or "regroup_kernel" in k)
+    # the staged schedule's regroup of K steps at once
+    total += sum(byt(v) for k, v in kern.items() if "regroup_multi_kernel" in k and k.endswith(f", {K}>")) / K
     scat = [(k, v) for k, v in kern.items() if "scatter_kernel" in k and (K == 1 or k.endswith(f", {K}>"))]
     if scat:
         k, v = scat[0]

@@ -210,8 +210,9 @@ int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_
  * leviathan_step, run singly) share one scatter pass: the first step of a
  * block gathers the delayed phases of all K steps, and step k of the block
  * first gathers only its edges with delay < k (their phases do not exist
- * before).  Phases, weights and r are bit-identical to K = 1.  K is 1, 2 or
- * 4; the handle keeps K bucketed value arrays (K * E values).
+ * before).  The staged schedule regroups the K steps' values in one pass too.
+ * Phases, weights and r are bit-identical to K = 1.  K is 1, 2 or 4; the
+ * handle keeps K arrays of E values (twice that for the staged schedule).
  * leviathan_get_scatter_steps returns K or -1.
  * leviathan_autotune_scatter_steps times K = 1, 2, 4 on the active plan from
  * the same state (ms per step to ms_out[3]), restores the state and keeps the

@@ -81,7 +81,8 @@ struct DevPlan {
   uint16_t* d_rg_rank = nullptr;
   int2* d_rg_runs = nullptr;
   int32_t n_rg_chunks = 0;
-  uint32_t* d_early_pos = nullptr;  // two-phase plans: value slot of each early edge (multi-step scatter)
+  // two-phase plans: each early edge's slot in the array the row pass reads (multi-step scatter)
+  uint32_t* d_early_pos = nullptr;
   bool owns_pos = false;
 };
 
@@ -134,7 +135,7 @@ struct LeviathanHandle {
   uint32_t* d_early_pos = nullptr;  // binned plans' value slots of those edges
   int32_t early_cnt[lev::kMaxScatterSteps] = {};  // early_cnt[k] = edges with delay < k
   int scatter_steps = 1;   // consecutive steps one scatter pass gathers (two-phase schedules)
-  int hbuf_steps = 1;      // steps d_hbuf has room for
+  int hbuf_steps = 1;      // steps d_hbuf (and the staged schedule's d_hbuf2) have room for
   int4* d_sources = nullptr;  // phase-1 work items
   int32_t n_sources = 0;
   int32_t n_buckets = 0;
@@ -350,15 +351,18 @@ int block_steps(const LeviathanHandle* h) {
 // Enqueue the step kernel(s) of the active plan, without the finalize, for
 // step k of a block of K consecutive steps whose values one scatter pass
 // gathers (K = 1: a step on its own).  Step 0 of a block runs the scatter
-// for all K steps; step k > 0 first gathers its edges with delay < k.
+// (and, staged, the regroup) for all K steps; step k > 0 first gathers its
+// edges with delay < k, straight into the array the row pass reads.
 // mid / mid2 (optional) are recorded after the scatter phase and after the regroup.
 hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t mid = nullptr,
                         hipEvent_t mid2 = nullptr) {
   const DevPlan& pl = h->plans[h->active];
   if (pl.kind == LEV_SCHEDULE_DIRECT)
     return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
-  const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  void* hk = static_cast<char*>(h->d_hbuf) + (size_t)k * (size_t)h->E * hist_bytes;
+  const bool staged = pl.kind == LEV_SCHEDULE_STAGED;
+  const size_t step_bytes = (size_t)h->E * (h->cfg.history == LEV_HISTORY_F32 ? 4 : 2);
+  // the array step k's row pass reads: bucketed values (binned) or regrouped ones (staged)
+  void* hk = static_cast<char*>(staged ? h->d_hbuf2 : h->d_hbuf) + (size_t)k * step_bytes;
   hipError_t e = hipSuccess;
   if (k == 0) {
     lev::ScatterArgs sa = scatter_args(h, pl);
@@ -378,22 +382,23 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     e = lev::launch_early(h->cfg.history, h->cfg.layout, ea, h->stream);
   }
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
-  if (e == hipSuccess && pl.kind == LEV_SCHEDULE_STAGED) {
+  if (e == hipSuccess && staged && k == 0) {
     lev::RegroupArgs ra{};
     ra.chunks = pl.d_rg_chunks;
     ra.rank = pl.d_rg_rank;
     ra.runs = pl.d_rg_runs;
-    ra.src = hk;
+    ra.src = h->d_hbuf;
     ra.dst = h->d_hbuf2;
     ra.num_chunks = pl.n_rg_chunks;
+    ra.stride = h->E;
+    ra.steps = K;
     e = lev::launch_regroup(h->cfg.history, ra, h->stream);
   }
   if (e == hipSuccess && mid2) e = hipEventRecord(mid2, h->stream);
   if (e == hipSuccess) {
     StepArgs a = step_args(h, dt, pl);
-    if (pl.kind == LEV_SCHEDULE_BINNED) a.hbuf = hk;
-    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1, a,
-                         h->stream);
+    a.hbuf = hk;
+    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, staged ? 2 : 1, a, h->stream);
   }
   return e;
 }
@@ -720,8 +725,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       }
       lev::sort_scatter_writes(hsc, st_pos[q], st_rank[q]);
       lev::encode_scatter_runs(hsc, st_pos[q], st_runs[q]);
+      // early edges are gathered straight into the regrouped array: their staged slots
       st_early_pos[q].resize(early_csr.size());
-      for (size_t i = 0; i < early_csr.size(); ++i) st_early_pos[q][i] = bhpos[early_csr[i]];
+      for (size_t i = 0; i < early_csr.size(); ++i) st_early_pos[q][i] = hpos[early_csr[i]];
       DevPlan& ps = h->plans[kPlanStaged + q];
       ps.built = true;
       ps.kind = LEV_SCHEDULE_STAGED;
@@ -1475,16 +1481,24 @@ bool time_trial(LeviathanHandle* h, StateSnapshot& snap, float dt, int steps, hi
 bool resize_hbuf(LeviathanHandle* h, int steps) {
   if (steps <= h->hbuf_steps) return true;
   const size_t one = std::max<size_t>((size_t)h->E * (h->cfg.history == LEV_HISTORY_F32 ? 4 : 2), 1);
-  void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, one * (size_t)steps);
-  if (e != hipSuccess) {
-    set_err("hipMalloc(%zu bytes) for the values of %d steps failed: %s", one * (size_t)steps, steps,
-            hipGetErrorString(e));
-    return false;
+  void* bufs[2] = {nullptr, nullptr};
+  const int nb = h->d_hbuf2 ? 2 : 1;  // the regrouped values of the staged schedule too
+  for (int i = 0; i < nb; ++i) {
+    const hipError_t e = hipMalloc(&bufs[i], one * (size_t)steps);
+    if (e != hipSuccess) {
+      set_err("hipMalloc(%zu bytes) for the values of %d steps failed: %s", one * (size_t)steps, steps,
+              hipGetErrorString(e));
+      if (bufs[0]) (void)hipFree(bufs[0]);
+      return false;
+    }
   }
   (void)hipFree(h->d_hbuf);
-  h->vram += one * (size_t)(steps - h->hbuf_steps);
-  h->d_hbuf = p;
+  h->d_hbuf = bufs[0];
+  if (nb == 2) {
+    (void)hipFree(h->d_hbuf2);
+    h->d_hbuf2 = bufs[1];
+  }
+  h->vram += (size_t)nb * one * (size_t)(steps - h->hbuf_steps);
   h->hbuf_steps = steps;
   return true;
 }

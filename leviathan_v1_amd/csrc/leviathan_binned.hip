@@ -321,13 +321,131 @@ __global__ __launch_bounds__(kBlock) void regroup_kernel(RegroupArgs a) {
   }
 }
 
+// Regroup of the values of K consecutive steps at once (several steps per
+// scatter pass): one chunk per workgroup.  The rank, the run heads' scan and
+// each position's staged slot are shared by the K steps; the steps' values are
+// all loaded first, then placed and written in turn through two LDS buffers
+// (one barrier per step).
+template <class H, int K>
+__global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
+  constexpr int PER = kRegroupChunk / kBlock;
+  __shared__ H s_val[2][kRegroupChunk];
+  __shared__ uint16_t s_run[kRegroupChunk];  // run heads, then each position's run
+  __shared__ int2 s_runs[kRegroupRuns];
+  __shared__ int32_t s_wsum[kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const H* __restrict__ src = reinterpret_cast<const H*>(a.src);
+  H* __restrict__ dst = reinterpret_cast<H*>(a.dst);
+  const int4 ch = a.chunks[blockIdx.x];
+  const int32_t n = ch.y, r0 = ch.z, nr = ch.w;
+
+  // unconditional loads: a lane past the chunk reads element 0 and discards it
+  H v[K][PER];
+  int32_t rk[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int32_t l = k * kBlock + tid;
+    const int64_t c = l < n ? (int64_t)ch.x + l : 0;
+    rk[k] = a.rank[c];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q][k] = src[q * a.stride + c];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int32_t l = k * kBlock + tid;
+    if (l < n) s_run[l] = 0;
+  }
+  for (int32_t r = tid; r < nr; r += kBlock) s_runs[r] = a.runs[r0 + r];
+  __syncthreads();
+  for (int32_t r = tid; r < nr; r += kBlock) s_run[s_runs[r].x] = 1;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int32_t l = k * kBlock + tid;
+    if (l < n) s_val[0][rk[k]] = v[0][k];
+  }
+  __syncthreads();
+
+  // exclusive scan of the heads: lane tid owns positions [PER * tid, PER * tid + PER)
+  int32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int32_t l = PER * tid + j;
+    cnt += (l < n) ? (int32_t)s_run[l] : 0;
+  }
+  int32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) s_wsum[wave] = incl;
+  __syncthreads();
+  int32_t run = incl - cnt;
+  for (int w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int32_t l = PER * tid + j;
+    if (l < n) {
+      run += s_run[l];
+      s_run[l] = (uint16_t)(run - 1);
+    }
+  }
+  __syncthreads();
+
+  // each position's staged slot, shared by the K steps
+  int32_t off[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int32_t l = k * kBlock + tid;
+    off[k] = 0;
+    if (l < n) {
+      const int2 rr = s_runs[s_run[l]];
+      off[k] = rr.y + (l - rr.x);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (q > 0) {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int32_t l = k * kBlock + tid;
+        if (l < n) s_val[q & 1][rk[k]] = v[q][k];
+      }
+      __syncthreads();
+    }
+    H* out = dst + q * a.stride;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) out[off[k]] = s_val[q & 1][l];
+    }
+  }
+}
+
+template <int K>
+static void regroup_multi_launch(int history, const RegroupArgs& a, hipStream_t s) {
+  const dim3 grid(a.num_chunks), block(kBlock);
+  if (history == 0)
+    regroup_multi_kernel<float, K><<<grid, block, 0, s>>>(a);
+  else
+    regroup_multi_kernel<uint16_t, K><<<grid, block, 0, s>>>(a);
+}
+
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
   if (a.num_chunks <= 0) return hipSuccess;
-  const dim3 grid((a.num_chunks + kRegroupChunksPerWg - 1) / kRegroupChunksPerWg), block(kBlock);
-  if (history == 0)
-    regroup_kernel<float><<<grid, block, 0, s>>>(a);
-  else
-    regroup_kernel<uint16_t><<<grid, block, 0, s>>>(a);
+  switch (a.steps) {
+    case 1: {
+      const dim3 grid((a.num_chunks + kRegroupChunksPerWg - 1) / kRegroupChunksPerWg), block(kBlock);
+      if (history == 0)
+        regroup_kernel<float><<<grid, block, 0, s>>>(a);
+      else
+        regroup_kernel<uint16_t><<<grid, block, 0, s>>>(a);
+      break;
+    }
+    case 2: regroup_multi_launch<2>(history, a, s); break;
+    case kMaxScatterSteps: regroup_multi_launch<kMaxScatterSteps>(history, a, s); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

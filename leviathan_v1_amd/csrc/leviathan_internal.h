@@ -108,6 +108,8 @@ struct RegroupArgs {
   const void* src;           // bucketed values
   void* dst;                 // staged values
   int32_t num_chunks;
+  int64_t stride;            // elements between consecutive steps' arrays (steps > 1)
+  int32_t steps;             // steps whose values one pass regroups (1, 2 or kMaxScatterSteps)
 };
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
 
@@ -143,13 +145,13 @@ struct ScatterArgs {
 constexpr int kMaxScatterSteps = 4;
 struct EarlyArgs {
   const int32_t* src;       // [count] source node
-  const uint32_t* pos;      // [count] value slot in the step's bucketed array
+  const uint32_t* pos;      // [count] value slot in that array
   const uint8_t* delay;     // [count] delay, ascending
   int32_t count;            // the records with delay < k
   const void* ring;
   int64_t ring_pitch;
   int32_t n_slots;
-  void* hbuf;               // the bucketed values of the step being prepared
+  void* hbuf;               // the values the step's row pass reads (bucketed, or regrouped when staged)
   const int64_t* step;
 };
 

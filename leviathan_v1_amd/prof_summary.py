@@ -22,7 +22,7 @@ from collections import defaultdict
 
 def _short(name: str) -> str:
     return (name.replace("(lev::StepArgs)", "").replace("(lev::ScatterArgs)", "").replace("(lev::EarlyArgs)", "")
-            .replace("void ", "").strip())
+            .replace("(lev::RegroupArgs)", "").replace("void ", "").strip())
 
 
 def stats(path, cmd, top=8):
