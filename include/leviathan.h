@@ -213,6 +213,7 @@ int leviathan_autotune(LeviathanHandle* handle, float dt, int steps, double* ms_
  * before).  The staged schedule regroups the K steps' values in one pass too.
  * Phases, weights and r are bit-identical to K = 1.  K is 1, 2 or 4; the
  * handle keeps K arrays of E values (twice that for the staged schedule).
+ * K may not exceed the ring's max_delay + 2 slots.
  * leviathan_get_scatter_steps returns K or -1.
  * leviathan_autotune_scatter_steps times K = 1, 2, 4 on the active plan from
  * the same state (ms per step to ms_out[3]), restores the state and keeps the

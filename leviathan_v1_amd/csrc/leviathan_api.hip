@@ -1557,6 +1557,10 @@ int leviathan_set_scatter_steps(LeviathanHandle* h, int steps) {
     set_err("no two-phase schedule was built for this handle");
     return -1;
   }
+  if (steps > h->n_slots) {  // the pass reads slots t - d .. t + K - 1 - d of the S-slot ring
+    set_err("%d steps per scatter pass exceed the ring's %d slots (max_delay + 2)", steps, h->n_slots);
+    return -1;
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
     set_err("hipStreamSynchronize failed");
     return -1;
@@ -1594,6 +1598,7 @@ int leviathan_autotune_scatter_steps(LeviathanHandle* h, float dt, int steps, do
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
   for (int i = 0; i < 3 && ok; ++i) {
+    if (cand[i] > h->n_slots) continue;  // more steps than ring slots
     if (!resize_hbuf(h, cand[i])) {
       snap.finish();
       if (e0) (void)hipEventDestroy(e0);

@@ -43,7 +43,8 @@ namespace lev {
 constexpr int scatter_threads(int) { return kBlock; }
 
 // slot of theta(t + k - d) on a ring of S slots, cur = t % S, 0 <= d <= S - 2
-// and 0 <= k < kMaxScatterSteps
+// and 0 <= k < S (the host allows at most S steps per pass), so that
+// -S < cur + k - d < 2 * S
 __device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d, int32_t S) {
   int32_t slot = cur + k - d;
   slot += (slot < 0) ? S : 0;

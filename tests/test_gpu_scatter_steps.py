@@ -67,7 +67,7 @@ def make(wl, **kw):
 def _same_state(a, b, max_delay):
     assert np.array_equal(a.get_theta(), b.get_theta())
     assert np.array_equal(a.get_weights(), b.get_weights())
-    for d in (0, 1, 2, max_delay):
+    for d in sorted({0, min(1, max_delay), min(2, max_delay), max_delay}):
         assert np.array_equal(a.get_history(d), b.get_history(d)), d
 
 
@@ -142,3 +142,31 @@ def test_checkpoint_across_scatter_steps(ws_d0, tmp_path):
     _same_state(a, b, ws_d0.max_delay)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("max_delay", [0, 1, 2])
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
+def test_short_rings(max_delay, schedule):
+    """Rings of S = max_delay + 2 slots take at most S steps per pass; with delays 0..max_delay
+    every edge of every later step of a block is early."""
+    from leviathan_v1_amd import workloads
+
+    say("scatter steps short ring", max_delay, schedule)
+    wl = workloads.watts_strogatz_workload(2000, 20, 0.1, max(max_delay, 1), seed=40 + max_delay)
+    wl = dataclasses.replace(wl, max_delay=max_delay,
+                             delays=np.random.default_rng(max_delay).integers(0, max_delay + 1, wl.E)
+                             .astype(np.uint8))
+    for K in (2, 4):
+        b = make(wl, history="u16", schedule=schedule)
+        if K > max_delay + 2:
+            with pytest.raises(RuntimeError, match="exceed the ring"):
+                b.set_scatter_steps(K)
+            assert set(b.autotune_scatter_steps(DT, steps=4)) == {k for k in (1, 2, 4) if k <= max_delay + 2}
+            b.close()
+            continue
+        b.set_scatter_steps(K)
+        ref = make(wl, history="u16", schedule=schedule)
+        np.testing.assert_array_equal(ref.step_n(DT, 45), b.step_n(DT, 45))
+        _same_state(ref, b, max_delay)
+        ref.close()
+        b.close()
