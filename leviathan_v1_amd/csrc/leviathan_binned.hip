@@ -37,11 +37,6 @@
 
 namespace lev {
 
-// lanes of a scatter workgroup gathering K steps per pass (measured at BA
-// N = 10^7, u16, K = 4: 1.82 ms per pass with 256 lanes and 16 edges per lane
-// at 164 VGPRs, 1.92 ms with 512 lanes and 8 edges per lane at 92 VGPRs)
-constexpr int scatter_threads(int) { return kBlock; }
-
 // slot of theta(t + k - d) on a ring of S slots, cur = t % S, 0 <= d <= S - 2
 // and 0 <= k < S (the host allows at most S steps per pass), so that
 // -S < cur + k - d < 2 * S
@@ -57,9 +52,11 @@ __device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d,
 // d < k are read from slots this pass must not trust (not yet written for
 // step t + k); early_kernel overwrites them before that step's row pass.
 template <class H, int LAYOUT, int K>
-__global__ __launch_bounds__(scatter_threads(K)) void scatter_kernel(ScatterArgs a) {
-  // a source block holds up to CAP out-edges, EPT per lane
-  constexpr int T = scatter_threads(K);
+__global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
+  // a source block holds up to CAP out-edges, EPT per lane.  (With K = 4 a
+  // 512-lane workgroup, 8 edges per lane at 92 instead of 164 VGPRs, took
+  // 1.92 ms per pass at BA N = 10^7, u16, against 1.82 ms with 256 lanes.)
+  constexpr int T = kBlock;
   constexpr int CAP = kBlock * kScatterEpt;
   constexpr int EPT = CAP / T;
   __shared__ uint8_t s_esrc[CAP];
@@ -457,7 +454,7 @@ hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
 
 template <int K>
 static void scatter_launch(int history, int layout, const ScatterArgs& a, hipStream_t s) {
-  const dim3 grid(a.num_blocks), block(scatter_threads(K));
+  const dim3 grid(a.num_blocks), block(kBlock);
   if (history == 0) {
     if (layout == 0)
       scatter_kernel<float, 0, K><<<grid, block, 0, s>>>(a);
