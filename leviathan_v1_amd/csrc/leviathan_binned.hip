@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // compiler waited for every history load before issuing the next.
     // the block's runs of consecutive slots: slot-order index k of run r goes
     // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
-    const int32_t rb = a.run_ptr[b], nr = min(a.run_ptr[b + 1] - rb, kScatterRuns);
+    const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
     int32_t tb[kScatterRuns / T];
 #pragma unroll
     for (int j = 0; j < kScatterRuns / T; ++j) {
@@ -164,15 +164,33 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         }
       }
       __syncthreads();
-      // write in slot order: the values bound for one bucket leave as one contiguous run
+      // write in slot order: the values bound for one bucket leave as one contiguous run.
+      // The branch on the run count is uniform over the block: with a per-lane
+      // choice between s_base and run_base the compiler read both through one
+      // flat load of a selected pointer, and every store then waited for all
+      // earlier loads and stores (vmcnt(0)).
+      // All slots and values are read from LDS first, into registers of their
+      // own, and the stores follow: with one register set reused per store,
+      // each store's source registers were guarded by a wait for every
+      // earlier store (vmcnt(0)).
       H* out = hbuf + (int64_t)q * a.hbuf_stride;
+      if (nr_all <= kScatterRuns) {
+        int32_t at[EPT];
+        H val[EPT];
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * T + tid;
-        if (le < nnz) {
-          const int32_t r = (int32_t)(rk[k] >> 16);
-          const int32_t base = r < kScatterRuns ? s_base[r] : a.run_base[rb + r];
-          out[base + le] = sv[le];
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
+          at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
+          val[k] = sv[le];
+        }
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+          if (k * T + tid < nnz) out[at[k]] = val[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;
+          if (le < nnz) out[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = sv[le];
         }
       }
       if (q + 1 < K) {
