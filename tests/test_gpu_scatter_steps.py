@@ -170,3 +170,39 @@ def test_short_rings(max_delay, schedule):
         _same_state(ref, b, max_delay)
         ref.close()
         b.close()
+
+
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
+def test_ensemble_and_driven_feature_capture(schedule):
+    """An ensemble of 12 networks (the finalize grid) with the reservoir drive and feature capture
+    on: several steps per pass give the same r of every network, the same phases and the same
+    captured feature rows as one step per pass."""
+    from leviathan_v1_amd import LeviathanEngine, graph, workloads
+
+    say("scatter steps ensemble + features", schedule)
+    wls = [workloads.watts_strogatz_workload(400 + 30 * k, 20, 0.1, 6, seed=70 + k) for k in range(12)]
+    g, offs = graph.disjoint_union([w.graph for w in wls])
+    cat = lambda f: np.concatenate([getattr(w, f) for w in wls])
+    delays = np.random.default_rng(3).integers(0, 7, g.num_edges).astype(np.uint8)
+    args = (g.num_nodes, 6, g.row_ptr, g.col_idx, delays, cat("weights"), cat("theta"), cat("theta_hat"),
+            cat("omega"))
+    kw = dict(n_nets=12, net_offsets=offs, g0=list(np.linspace(0, 4, 12)), controller=[k % 2 == 0 for k in range(12)],
+              history="u16", schedule=schedule)
+    rng = np.random.default_rng(4)
+    drive_b = rng.normal(0, 0.3, g.num_nodes).astype(np.float32)
+    u = rng.normal(0, 1, 230).astype(np.float32)
+    nodes = np.sort(rng.choice(g.num_nodes, 64, replace=False)).astype(np.int32)
+    engines = []
+    for K in (1, 4):
+        e = LeviathanEngine(*args, **kw)
+        if K > 1:
+            e.set_scatter_steps(K)
+        e.set_drive(drive_b)
+        e.features_setup(nodes, 300)
+        engines.append((e, e.step_n_features(DT, u=u)))
+    (a, ra), (b, rb) = engines
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(a.features_get(0, 230), b.features_get(0, 230))
+    assert np.array_equal(a.get_theta(), b.get_theta())
+    a.close()
+    b.close()
