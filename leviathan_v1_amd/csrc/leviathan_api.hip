@@ -1600,6 +1600,7 @@ int leviathan_autotune_scatter_steps(LeviathanHandle* h, float dt, int steps, do
   for (int i = 0; i < 3 && ok; ++i) {
     if (cand[i] > h->n_slots) continue;  // more steps than ring slots
     if (!resize_hbuf(h, cand[i])) {
+      h->scatter_steps = k0;
       snap.finish();
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
