@@ -72,22 +72,22 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
                           schedule=schedule, bucket_edges=bucket_edges)
     log(f"{history}: engine ready in {time.perf_counter() - t0:.1f} s")
     try:
-        tuned = eng.autotune(wl.dt, steps=5) if schedule == "auto" else None
+        tuned = k_ms = None
+        if schedule == "auto" and scatter_steps == "auto":
+            # plans, steps per scatter pass of the best two-phase plan, plans again at that K
+            res = eng.autotune_all(wl.dt, steps=20)
+            tuned, k_ms = res.get("plans_at_k", res["plans"]), res.get("scatter_steps")
+            log(f"{history}: autotune {res} -> {eng.get_schedule()}, {eng.scatter_steps} steps per pass")
+        else:
+            tuned = eng.autotune(wl.dt, steps=5) if schedule == "auto" else None
+            if eng.get_schedule()[0] != "direct":
+                if scatter_steps == "auto":
+                    k_ms = eng.autotune_scatter_steps(wl.dt, steps=8)
+                else:
+                    eng.set_scatter_steps(int(scatter_steps))
+            log(f"{history}: autotune {tuned}, scatter steps {k_ms} -> {eng.get_schedule()}, "
+                f"{eng.scatter_steps} steps per pass")
         chosen = eng.get_schedule()
-        log(f"{history}: autotune {tuned} -> {chosen}")
-        # steps per scatter pass (two-phase schedules; the direct one has no scatter)
-        k_ms = None
-        if chosen[0] != "direct":
-            if scatter_steps == "auto":
-                k_ms = eng.autotune_scatter_steps(wl.dt, steps=8)
-            else:
-                eng.set_scatter_steps(int(scatter_steps))
-            log(f"{history}: scatter steps {k_ms} -> {eng.scatter_steps}")
-            if schedule == "auto" and eng.scatter_steps > 1:
-                # the plans again, the two-phase ones at that many steps per pass
-                tuned = eng.autotune(wl.dt, steps=8)
-                chosen = eng.get_schedule()
-                log(f"{history}: autotune at {eng.scatter_steps} steps per pass {tuned} -> {chosen}")
         eng.step_n(wl.dt, warmup)
         t0 = time.perf_counter()
         r = eng.step_n(wl.dt, steps)  # returns after the stream is synchronised

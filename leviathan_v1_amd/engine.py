@@ -409,6 +409,25 @@ class LeviathanEngine:
             _lib.check("autotune_scatter_steps")
         return {k: float(v) for k, v in zip((1, 2, 4), out) if v == v}
 
+    def autotune_all(self, dt: float, steps: int = 8) -> dict:
+        """The plans, then the steps per scatter pass of the fastest two-phase plan, then the plans
+        again (the two-phase ones at that many steps per pass).  Each timing starts from the current
+        state, which is restored.  Returns {"plans": ms per plan, "scatter_steps": ms per K,
+        "plans_at_k": ms per plan at the chosen K (absent if K stayed 1)}."""
+        out = {"plans": self.autotune(dt, steps)}
+        two_phase = {k: v for k, v in out["plans"].items() if not k.startswith("direct")}
+        if not two_phase:
+            return out
+        chosen = self.get_schedule()
+        best = min(two_phase, key=two_phase.get)
+        self.set_schedule(best.split("/")[0], int(best.split("/")[1]))
+        out["scatter_steps"] = self.autotune_scatter_steps(dt, steps)
+        if self.scatter_steps > 1:
+            out["plans_at_k"] = self.autotune(dt, steps)
+        else:
+            self.set_schedule(*chosen)
+        return out
+
     def close(self) -> None:
         if self._h is not None:
             _lib.lib().leviathan_free(self._h)

@@ -32,9 +32,11 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
     _log(f"{count} networks, N={N} E={E}, built in {time.perf_counter() - t0:.1f} s")
 
     ens = LeviathanEnsemble(wls, spec.gains, history=history, layout=layout, schedule="auto", graph_chunk=100)
-    tuned = ens.engine.autotune(ens.dt, steps=5)
+    res = ens.engine.autotune_all(ens.dt, steps=20)
+    tuned = res.get("plans_at_k", res["plans"])
     chosen = ens.engine.schedule_name
-    _log("autotune", tuned, "->", chosen)
+    scatter_steps = ens.engine.scatter_steps if not chosen.startswith("direct") else None
+    _log("autotune", res, "->", chosen, scatter_steps)
     ens.step_n(warmup)
     t0 = time.perf_counter()
     r = ens.step_n(steps)
@@ -76,7 +78,7 @@ def run(steps: int = 200, warmup: int = 20, count: int = 256, seq_steps: int = 1
         "data": "synthetic (seeded graphs and initial states)",
         "config": {"workload": f"{count} Watts-Strogatz networks k=20 p=0.1, sizes log-uniform [2000, 200000], "
                                f"K=linspace(0,4), controller off", "N": N, "E": E, "history": history, "layout": layout,
-                   "schedule": chosen, "graph_chunk": 100},
+                   "schedule": chosen, "scatter_steps": scatter_steps, "graph_chunk": 100},
         "autotune_ms_per_step": tuned,
         "step_kernel_ms": prof["step_kernel_ms"] + prof["scatter_kernel_ms"] + prof["regroup_kernel_ms"],
         "sequential_plain": {"value": sequential, "unit": "edge-steps/s", "steps": seq_steps,
