@@ -206,3 +206,25 @@ def test_ensemble_and_driven_feature_capture(schedule):
     assert np.array_equal(a.get_theta(), b.get_theta())
     a.close()
     b.close()
+
+
+def test_autotune_all_restores_state(ba50k):
+    """Plans, steps per pass and plans again, each from the same state: the state is restored,
+    and the chosen configuration steps like a direct engine (every plan and every K gives the
+    same phases and weights)."""
+    say("autotune_all")
+    a = make(ba50k, history="u16", schedule="auto")
+    ref = make(ba50k, history="u16")
+    a.step_n(DT, 6)
+    ref.step_n(DT, 6)
+    res = a.autotune_all(DT, steps=4)
+    say("autotune_all", res, a.schedule_name, a.scatter_steps)
+    assert len(res["plans"]) == 9 and set(res["scatter_steps"]) == {1, 2, 4}
+    assert a.step_count == 6
+    _same_state(a, ref, ba50k.max_delay)
+    a.step_n(DT, 24)
+    ref.step_n(DT, 24)
+    assert np.array_equal(a.get_weights(), ref.get_weights())
+    assert np.array_equal(a.get_theta(), ref.get_theta())
+    a.close()
+    ref.close()
