@@ -340,12 +340,13 @@ __global__ __launch_bounds__(kBlock) void regroup_kernel(RegroupArgs a) {
 // Regroup of the values of K consecutive steps at once (several steps per
 // scatter pass): one chunk per workgroup.  The rank, the run heads' scan and
 // each position's staged slot are shared by the K steps; the steps' values are
-// all loaded first, then placed and written in turn through two LDS buffers
-// (one barrier per step).
+// all loaded first, then placed and written in turn through one LDS buffer
+// (two barriers per step; a second buffer would cost a sixth of the
+// workgroups per CU, and the pass was slower with it).
 template <class H, int K>
 __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   constexpr int PER = kRegroupChunk / kBlock;
-  __shared__ H s_val[2][kRegroupChunk];
+  __shared__ H s_val[kRegroupChunk];
   __shared__ uint16_t s_run[kRegroupChunk];  // run heads, then each position's run
   __shared__ int2 s_runs[kRegroupRuns];
   __shared__ int32_t s_wsum[kBlock / 64];
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int32_t l = k * kBlock + tid;
-    if (l < n) s_val[0][rk[k]] = v[0][k];
+    if (l < n) s_val[rk[k]] = v[0][k];
   }
   __syncthreads();
 
@@ -427,10 +428,11 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     if (q > 0) {
+      __syncthreads();  // the previous step's values have left the LDS
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int32_t l = k * kBlock + tid;
-        if (l < n) s_val[q & 1][rk[k]] = v[q][k];
+        if (l < n) s_val[rk[k]] = v[q][k];
       }
       __syncthreads();
     }
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int32_t l = k * kBlock + tid;
-      if (l < n) out[off[k]] = s_val[q & 1][l];
+      if (l < n) out[off[k]] = s_val[l];
     }
   }
 }
