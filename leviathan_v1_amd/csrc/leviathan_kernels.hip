@@ -104,8 +104,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     // ---------------- stream mode ----------------
     RowState rs_;
     int32_t rs = 0, re = 0;
+    float om = 0.0f;  // natural frequency (plus drive), loaded with the row's state: a load after
+                      // the last barrier would add a memory round trip to every row block
     if (tid < nrows) {
       const int32_t i = r0 + tid;
+      om = a.omega[i];
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
       s_theta[tid] = rs_.th;
       s_gate[tid] = rs_.gate;
@@ -219,8 +223,6 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 
     if (tid < nrows) {
       const int32_t i = r0 + tid;
-      float om = a.omega[i];
-      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
@@ -235,7 +237,10 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     // ---------------- long-row mode: one row, whole workgroup ----------------
     const int32_t i = r0;
     RowState rs_;
+    float om = 0.0f;
     if (tid == 0) {
+      om = a.omega[i];
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
       s_theta[0] = rs_.th;
       s_gate[0] = rs_.gate;
@@ -293,8 +298,6 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       double tot = 0.0;
 #pragma unroll
       for (int q = 0; q < kBlock / 64; ++q) tot += s_red[0][q];
-      float om = a.omega[i];
-      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       const float x = row_integrate(rs_, om, g, (float)tot, dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
