@@ -204,13 +204,15 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     }
     __syncthreads();
 
-    // Row sums in CSR order: Neumaier's sum by the row's lane, or for a long
-    // row lane-strided Neumaier partials combined by a fixed FP64 tree.
+    // Row sums in CSR order: an FP64 sum by the row's lane, rounded once (two
+    // VALU instructions per term where Neumaier's FP32 sum took about eight, in
+    // a loop the wave runs for its longest row), or for a long row lane-strided
+    // Neumaier partials combined by a fixed FP64 tree.
     float coupling = 0.0f;
     if (tid < nrows && !heavy) {
-      CompSum acc;
-      for (int32_t q = rs; q < re; ++q) acc.add(s_contrib[q]);
-      coupling = acc.value();
+      double acc = 0.0;
+      for (int32_t q = rs; q < re; ++q) acc += (double)s_contrib[q];
+      coupling = (float)acc;
     }
     for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
       const int src = __ffsll((unsigned long long)m) - 1;
