@@ -23,7 +23,7 @@ __host__ __device__ inline int4 pack_item(int32_t r0, int32_t e0, int32_t e1, in
                                           bool is_long) {
   return make_int4(r0, e0, e1, (net << 10) | (nrows << 1) | (is_long ? kBlockLongRow : 0));
 }
-constexpr int kSerialRow = 32;       // stream blocks: longer rows are filled and summed by a wavefront
+constexpr int kSerialRow = 128;      // stream blocks: longer rows are filled and summed by a wavefront
 // Estimated row-pass cost of a block (ns of workgroup lifetime), for balancing the XCDs.
 constexpr double kCostBlock = 9860.0, kCostEdge = 7.42, kCostRow = 11.9, kCostHeavyRow = 300.0;
 constexpr double kCostLong = 20900.0, kCostLongEdge = 5.59;
