@@ -191,6 +191,9 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int k = 0; k < EPT; ++k) hh[k] = HistCodec<H>::decode(hr[k]);
     }
     if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
+    // one base address for the lane's weight stores (each store adds a constant
+    // offset), so that no store recomputes a 64-bit address
+    float* const wlane = a.w + e0 + tid;
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         float c;
         const float wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
         s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
-        a.w[e0 + le] = wn;
+        wlane[k * kBlock] = wn;
       }
     }
     __syncthreads();

@@ -144,8 +144,11 @@ __device__ __forceinline__ float edge_update(float h, float th_i, float gate, fl
   sincos_fast(diff, &sn, &cs);
   *contrib = w * sn;
   float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
+  // NaN and infinities to w_min, then the clamp as compares and selects (fminf
+  // and fmaxf quieted both bounds again for every edge)
   if (!__builtin_isfinite(wv)) wv = w_min;
-  return fmaxf(w_min, fminf(w_max, wv));
+  wv = wv > w_max ? w_max : wv;
+  return wv < w_min ? w_min : wv;
 }
 
 // Logical row-block index for a workgroup.  Workgroups are dealt to the 8
