@@ -227,8 +227,8 @@ def main():
     if args.workload == "reservoir":
         from leviathan_v1_amd import reservoir_bench
 
-        out = reservoir_bench.run(steps=max(args.steps, 2000), oracle_check=not args.no_cpu,
-                                  static=args.static_reservoir, graph_chunk=args.graph_chunk)
+        out = reservoir_bench.run(steps=max(args.steps, 2000), static=args.static_reservoir,
+                                  graph_chunk=args.graph_chunk)
         print(json.dumps({"metric": "reservoir readout NRMSE (u(t+10)), BF16 MFMA Gram + FP64 refinement",
                           "value": out["nrmse_test"], "unit": "NRMSE", "n_gpus": 1, "steps": max(args.steps, 2000),
                           "higher_is_better": False, "data": "synthetic", **out}))

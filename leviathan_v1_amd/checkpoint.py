@@ -19,9 +19,14 @@ FIELDS = ("theta", "theta_hat", "omega_hat", "weights", "history", "gains", "ste
 
 
 def graph_digest(row_ptr, col_idx, delays) -> str:
+    """SHA-256 of the CSR arrays and delays in the engine's own types (int32, int32, uint8),
+    so the same graph digests the same whatever integer types the caller holds it in."""
     h = hashlib.sha256()
-    for a in (row_ptr, col_idx, delays):
-        h.update(np.ascontiguousarray(a).tobytes())
+    for a, t in ((row_ptr, np.int32), (col_idx, np.int32), (delays, np.uint8)):
+        a = np.asarray(a)
+        if a.size and a.dtype != t and not np.array_equal(a, a.astype(t)):
+            raise ValueError(f"graph array does not fit {np.dtype(t).name}")
+        h.update(memoryview(np.ascontiguousarray(a, dtype=t)).cast("B"))
     return h.hexdigest()
 
 
@@ -33,6 +38,8 @@ def save(path, state: dict, row_ptr=None, col_idx=None, delays=None) -> None:
     extra = {}
     if row_ptr is not None:
         extra["graph_digest"] = np.array(graph_digest(row_ptr, col_idx, delays))
+    elif "graph_digest" in state:
+        extra["graph_digest"] = np.asarray(state["graph_digest"])
     np.savez(path, **{k: np.asarray(state[k]) for k in FIELDS}, **extra)
 
 

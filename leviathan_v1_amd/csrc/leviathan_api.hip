@@ -216,7 +216,12 @@ bool upload_runs(LeviathanHandle* h, const std::vector<uint16_t>& rank, const le
   return true;
 }
 
+// Destroys the captured graphs.  A graph launched earlier may still be running
+// on the handle's stream, so the stream is drained first: destroying an
+// executable graph in flight would free the kernel arguments it still reads.
 void drop_graphs(LeviathanHandle* h) {
+  if (h->graphs.empty()) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
   h->graphs.clear();
 }
@@ -1491,6 +1496,15 @@ bool resize_hbuf(LeviathanHandle* h, int steps) {
       if (bufs[0]) (void)hipFree(bufs[0]);
       return false;
     }
+  }
+  // the captured graphs hold the old arrays' addresses: drop them before the
+  // arrays are freed (drop_graphs drains the stream first)
+  drop_graphs(h);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_err("hipStreamSynchronize failed");
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    return false;
   }
   (void)hipFree(h->d_hbuf);
   h->d_hbuf = bufs[0];

@@ -19,6 +19,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
+from . import checkpoint as _checkpoint
 
 _PARAM_NAMES = ("eta", "lam", "gamma", "alpha_eps", "kappa_w", "w_min", "w_max", "g0", "r0", "beta", "mu",
                 "g_rest", "g_min", "g_max")
@@ -79,6 +80,8 @@ class LeviathanEngine:
         omega = _arr(omega, np.float32, N, "omega")
         self.N, self.E, self.max_delay = N, E, int(max_delay)
         self.n_nets = int(n_nets)
+        # identifies the graph in checkpoints (restore refuses another graph's state)
+        self.graph_digest = _checkpoint.graph_digest(row_ptr, col_idx, delays)
         extended = (history != "f32" or layout != "slot" or edges_per_thread != 8 or graph_chunk != 100
                     or schedule != "direct" or bucket_edges
                     or params or n_nets != 1 or net_offsets is not None or g0 is not None
@@ -245,12 +248,15 @@ class LeviathanEngine:
             hist[d - 1] = self.get_history(d)
         return {"theta": self.get_theta(), "theta_hat": thh, "omega_hat": omh, "weights": self.get_weights(),
                 "history": hist, "gains": self.get_gains(), "step": np.int64(self.step_count),
-                "N": np.int64(self.N), "E": np.int64(self.E), "max_delay": np.int64(D)}
+                "N": np.int64(self.N), "E": np.int64(self.E), "max_delay": np.int64(D),
+                "graph_digest": np.array(self.graph_digest)}
 
     def restore(self, state: dict) -> None:
         """Continue from a checkpoint() of an engine with the same graph and delays."""
         if (int(state["N"]), int(state["E"]), int(state["max_delay"])) != (self.N, self.E, self.max_delay):
             raise ValueError("checkpoint is of a different graph (N, E or max_delay differ)")
+        if "graph_digest" in state and str(state["graph_digest"]) != self.graph_digest:
+            raise ValueError("checkpoint was saved from a different graph (CSR or delays differ)")
         self.set_step_count(int(state["step"]))
         self.reset_weights(state["weights"])
         self.set_theta(state["theta"])

@@ -9,6 +9,9 @@ Three restatements of the reference step (src/cuda/leviathan_kernel.cu:391-438):
 
 * ``NumpyOracle``  -- FP64, vectorised NumPy (this file).
 * ``COracle(dtype=np.float64)`` -- FP64 C, sequential (leviathan_oracle.c).
+* ``COracle(dtype=np.float64, parallel=True)`` -- the same FP64 C with OpenMP
+  over rows and nodes; theta, W and g bit-identical to the sequential one, r
+  summed in a fixed blocked order (for the 10^7-node workload).
 * ``COracle(dtype=np.float32)`` -- FP32 C/OpenMP port of the reference's four
   kernels: the CPU baseline.
 
@@ -232,13 +235,13 @@ class COracle(_StateMixin):
     """
 
     def __init__(self, N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega,
-                 params=None, dtype=np.float64):
+                 params=None, dtype=np.float64, parallel=False):
         self._init_state(N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega, params, dtype)
         lib = _load()
-        sfx = "f64" if self.dtype == np.float64 else "f32"
+        sfx = ("f64p" if parallel else "f64") if self.dtype == np.float64 else "f32"
         self._run = getattr(lib, "lev_oracle_run_" + sfx)
         self._run.restype = None
-        self._real = ctypes.c_double if sfx == "f64" else ctypes.c_float
+        self._real = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
         self._bi = ctypes.c_int32(0)
         self._g = np.array([self.g], dtype=self.dtype)
         self.drive_b = None

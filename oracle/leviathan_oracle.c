@@ -4,9 +4,18 @@
  * file: it is the checker for the gfx950 kernels (tests/, smoke(), and the
  * cpu_baseline leg of bench.py) and the FP32 CPU baseline.
  *
- * Two instantiations of one body (oracle_step.inc):
+ * Three instantiations of one body (oracle_step.inc):
  *   lev_oracle_step_f64  -- FP64 arithmetic, single thread, sequential edge
  *                           sums: the correctness oracle.
+ *   lev_oracle_step_f64p -- the same FP64 arithmetic with OpenMP over rows and
+ *                           nodes.  Every row's edge sum is still sequential in
+ *                           CSR order and every per-node value is computed as
+ *                           in lev_oracle_step_f64, so theta, W and g are
+ *                           bit-identical to it; only r is summed in a fixed
+ *                           blocked order (256-node block sums, then the block
+ *                           sums in order), independent of the thread count.
+ *                           For workloads too large for one core (the
+ *                           10^7-node graph).
  *   lev_oracle_step_f32  -- FP32 arithmetic, OpenMP over rows, mirroring the
  *                           reference's four-kernel FP32 step: the CPU
  *                           baseline ("port") timed beside the GPU.
@@ -26,6 +35,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 typedef struct {
   double eta, lam, gamma, alpha_eps, kappa_w;
@@ -33,6 +43,8 @@ typedef struct {
   double r0, beta, mu, g_rest, g_min, g_max;
   int32_t gain_controller;
 } lev_oracle_params;
+
+#define PREFETCH 0
 
 /* FP64 instantiation */
 #define REAL double
@@ -52,6 +64,30 @@ typedef struct {
 #undef SQRT
 #undef TWO_PI_R
 #undef USE_OMP
+
+/* FP64 instantiation, OpenMP */
+#define REAL double
+#define SUFFIX f64p
+#define SIN sin
+#define COS cos
+#define FLOOR floor
+#define SQRT sqrt
+#define TWO_PI_R 6.283185307179586476925286766559
+#define USE_OMP 1
+#undef PREFETCH
+#define PREFETCH 16
+#include "oracle_step.inc"
+#undef PREFETCH
+#undef REAL
+#undef SUFFIX
+#undef SIN
+#undef COS
+#undef FLOOR
+#undef SQRT
+#undef TWO_PI_R
+#undef USE_OMP
+
+#define PREFETCH 0
 
 /* FP32 instantiation */
 #define REAL float

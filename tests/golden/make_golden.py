@@ -8,6 +8,7 @@ its inputs; the inputs themselves are regenerated from the seed.
 
     python tests/golden/make_golden.py ws1k      # WS N=1,000: 500 steps
     python tests/golden/make_golden.py ws100k    # WS N=100,000: 10,000 steps (minutes)
+    python tests/golden/make_golden.py ba10m     # BA N=10^7 m=10: 2,000 steps (OpenMP, ~1 h)
 """
 import hashlib
 import os
@@ -65,7 +66,31 @@ def make_ws100k(steps=10_000):
                         r=r, g_final=np.array(o.g))
 
 
+def make_ba10m(steps=2_000):
+    """Barabasi-Albert N=10^7, m=10, delays U{1..50}, plasticity and the gain controller on:
+    r(t) and g(t) of 2,000 FP64 steps (the OpenMP FP64 oracle, whose phases and weights are
+    bit-identical to the sequential one).  Written in chunks, so an interrupted run resumes."""
+    wl = workloads.scale_skew(0)
+    dg = digest(wl)
+    print(f"ba10m: N={wl.N} E={wl.E} digest {dg[:16]}", flush=True)
+    o = oracle.COracle(*wl.engine_args(), dtype=np.float64, parallel=True)
+    t0 = time.time()
+    r = np.empty(steps)
+    g = np.empty(steps)
+    chunk = 50
+    for s in range(0, steps, chunk):
+        r[s:s + chunk] = o.run(DT, chunk)
+        g[s + chunk - 1] = o.g
+        print(f"ba10m: {s + chunk} steps, {time.time() - t0:.0f} s, r={r[s + chunk - 1]:.6f} g={o.g:.6f}",
+              flush=True)
+    g_mask = np.zeros(steps, bool)
+    g_mask[chunk - 1::chunk] = True
+    np.savez_compressed(os.path.join(HERE, "ba10m_fp64.npz"), digest=np.array(dg), dt=np.array(DT), r=r,
+                        g_steps=np.nonzero(g_mask)[0], g=g[g_mask], g_final=np.array(o.g),
+                        theta_mean_cos=np.array(np.cos(o.theta).mean()), seconds=np.array(time.time() - t0))
+
+
 if __name__ == "__main__":
     oracle.build()
     for name in sys.argv[1:] or ["ws1k"]:
-        {"ws1k": make_ws1k, "ws100k": make_ws100k}[name]()
+        {"ws1k": make_ws1k, "ws100k": make_ws100k, "ba10m": make_ba10m}[name]()

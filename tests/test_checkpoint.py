@@ -40,3 +40,17 @@ def test_incomplete_state_is_refused(tmp_path):
     del s["history"]
     with pytest.raises(ValueError):
         checkpoint.save(tmp_path / "c.npz", s)
+
+
+def test_digest_ignores_integer_types(tmp_path):
+    """The same graph held as int64 row pointers (scipy's indptr) and int32 columns digests
+    the same, so its checkpoint loads; values outside the engine's types are refused."""
+    s = _state()
+    rp, ci, dl = np.arange(51, dtype=np.int32), np.zeros(300, np.int32), np.ones(300, np.uint8)
+    assert checkpoint.graph_digest(rp, ci, dl) == checkpoint.graph_digest(rp.astype(np.int64), ci.astype(np.int64),
+                                                                         dl.astype(np.int32))
+    p = tmp_path / "c.npz"
+    checkpoint.save(p, s, rp.astype(np.int64), ci, dl)
+    checkpoint.load(p, rp, ci, dl)
+    with pytest.raises(ValueError):
+        checkpoint.graph_digest(rp.astype(np.int64) + 2**40, ci, dl)

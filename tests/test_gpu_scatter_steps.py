@@ -124,6 +124,26 @@ def test_scatter_steps_autotune_restores_state(ba50k):
         x.close()
 
 
+@pytest.mark.parametrize("schedule", ["binned", "staged"])
+def test_autotune_scatter_steps_after_captured_graphs(ba50k, schedule):
+    """step_n captures 100-step graphs that hold the two-phase value arrays' addresses.  The
+    K autotune grows those arrays (room for 4 steps) whatever K it keeps, so every captured
+    graph must be dropped with the old arrays: the next step_n replays graphs of the new
+    arrays and continues exactly like an engine that never autotuned."""
+    say("scatter steps autotune after captured graphs", schedule)
+    a = make(ba50k, history="u16", schedule=schedule)
+    ref = make(ba50k, history="u16", schedule=schedule)
+    np.testing.assert_array_equal(a.step_n(DT, 200), ref.step_n(DT, 200))  # two captured graphs each
+    a.autotune_scatter_steps(DT, steps=8)
+    ref.set_scatter_steps(a.scatter_steps)
+    say("kept K =", a.scatter_steps)
+    for n in (200, 100, 3):
+        np.testing.assert_array_equal(a.step_n(DT, n), ref.step_n(DT, n))
+    _same_state(a, ref, ba50k.max_delay)
+    a.close()
+    ref.close()
+
+
 def test_checkpoint_across_scatter_steps(ws_d0, tmp_path):
     """A checkpoint taken mid-run with K = 4 restores into an engine with K = 2 and continues
     bit for bit like the original."""

@@ -38,10 +38,10 @@ def tiny(kind, N, edges, theta, omega=None, theta_hat=None, max_delay=5, **pk):
             theta, theta_hat, omega)
     if kind == "numpy":
         return NumpyOracle(*args, params=params)
-    return COracle(*args, params=params, dtype=np.float64)
+    return COracle(*args, params=params, dtype=np.float64, parallel=kind == "c64p")
 
 
-KINDS = ["numpy", "c64"]
+KINDS = ["numpy", "c64", "c64p"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -232,6 +232,76 @@ def test_set_theta_resets_history(kind):
     o.set_theta(new)
     for k in range(o.max_delay + 1):
         assert np.array_equal(o.history(k), new)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_input_drive_closed_form(kind):
+    """The input drive omega_i(t) = omega_i + b_i*u_t (the reservoir's extension; the reference
+    has no input term) with g = 0 and alpha = 0: theta_i(t) = wrap(theta_i(0) +
+    dt*sum_{s<t}(omega_i + b_i*u_s)).  A drive read one step late, with the wrong sign, or
+    applied to the predictor instead of the phase fails it (u varies from step to step and
+    b has both signs)."""
+    th0 = np.array([0.3, 2.5, 5.0, 1.0])
+    om = np.array([0.4, -0.9, 1.5, 0.0])
+    b = np.array([1.0, -0.5, 0.25, 2.0])
+    u = np.sin(0.37 * np.arange(200)) + 0.1 * np.cos(1.3 * np.arange(200))
+    o = tiny(kind, 4, [(0, 1, 2, 0.7), (2, 3, 1, 0.4)], th0, om, g0=0.0, gain_controller=False, alpha_eps=0.0,
+             kappa_w=0.0)
+    o.drive_b = b.copy()
+    acc = th0.copy()
+    for t in range(200):
+        o.step(DT, u[t])
+        acc = acc + DT * (om + b * u[t])
+        assert circ_err(o.theta, np.remainder(acc, TWO_PI)).max() < 1e-11, t
+    # with kappa_w = 0 omega_hat stays the undriven omega: the predictor does not see the drive
+    assert np.allclose(o.theta_hat, th0 + 200 * DT * om, atol=1e-11, rtol=0)
+
+
+def test_numpy_and_c_agree_with_drive(oracle_lib):
+    """NumPy and both C restatements agree over 300 driven steps with plasticity, the gain
+    controller and the coupling on."""
+    from leviathan_v1_amd import workloads
+
+    wl = workloads.small_plain(5)
+    a = wl.engine_args()
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1, 1, wl.N)
+    u = rng.normal(0, 1, 300)
+    c = COracle(*a, dtype=np.float64)
+    cp = COracle(*a, dtype=np.float64, parallel=True)
+    n = NumpyOracle(*a)
+    for o in (c, cp, n):
+        o.drive_b = b.copy()
+    rc = c.run(DT, 300, u)
+    rp = cp.run(DT, 300, u)
+    rn = n.run(DT, 300, u)
+    assert np.abs(rc - rn).max() < 1e-12
+    assert circ_err(c.theta, n.theta).max() < 1e-10
+    assert np.abs(c.w - n.w).max() < 1e-12
+    assert abs(c.g - n.g) < 1e-13
+    # the OpenMP restatement: phases, weights and gain bit-identical, r summed in blocks
+    assert np.array_equal(c.theta, cp.theta) and np.array_equal(c.w, cp.w) and c.g == cp.g
+    assert np.abs(rc - rp).max() < 1e-14
+    # and the drive matters: the same run undriven ends elsewhere
+    c0 = COracle(*a, dtype=np.float64)
+    c0.run(DT, 300)
+    assert circ_err(c0.theta, c.theta).max() > 1e-2
+
+
+def test_parallel_fp64_oracle_bit_identical_on_ba(oracle_lib):
+    """The OpenMP FP64 restatement (used for the 10^7-node fixture) equals the sequential
+    one bit for bit in phases, weights and gain on a heavy-tailed graph (rows of hundreds of
+    edges, so the dynamic row schedule splits them across threads)."""
+    from leviathan_v1_amd import workloads
+
+    wl = workloads.barabasi_albert_workload(20_000, 10, 50, seed=4)
+    a = wl.engine_args()
+    c = COracle(*a, dtype=np.float64)
+    cp = COracle(*a, dtype=np.float64, parallel=True)
+    rc = c.run(DT, 40)
+    rp = cp.run(DT, 40)
+    assert np.array_equal(c.theta, cp.theta) and np.array_equal(c.w, cp.w) and c.g == cp.g
+    assert np.abs(rc - rp).max() < 1e-14
 
 
 def test_numpy_and_c_fp64_agree_on_small_workload(oracle_lib):
