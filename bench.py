@@ -111,22 +111,86 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         eng.close()
 
 
-def cpu_baseline(wl, budget_s=15.0, max_steps=50):
-    """FP32 OpenMP port of the reference's four-kernel step (oracle/), same graph and inputs."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, budget_s=10.0, max_steps=50):
+    """FP32 OpenMP port of the reference's four-kernel step (oracle/), same graph and inputs,
+    timed with all the threads OpenMP is given and with one thread."""
     import oracle
 
     oracle.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.COracle(*wl.engine_args(), dtype=np.float32)
     o.run(wl.dt, 1)  # first touch
-    n, t0 = 0, time.perf_counter()
-    while n < max_steps and time.perf_counter() - t0 < budget_s:
-        o.run(wl.dt, 1)
-        n += 1
-    el = time.perf_counter() - t0
-    return {"value": wl.E * n / el, "unit": "edge-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the same graph and inputs (FP32 C/OpenMP port of "
-                      f"leviathan_kernel.cu:93-264,391-438), {el:.1f} s"}
+
+    def rate(threads):
+        prev = oracle.set_threads(threads)
+        try:
+            o.run(wl.dt, 1)
+            n, t0 = 0, time.perf_counter()
+            while n < max_steps and time.perf_counter() - t0 < budget_s:
+                o.run(wl.dt, 1)
+                n += 1
+            return wl.E * n / (time.perf_counter() - t0), n, time.perf_counter() - t0
+        finally:
+            oracle.set_threads(prev)
+
+    threads = oracle.set_threads(1)
+    oracle.set_threads(threads)
+    all_rate, n_all, el_all = rate(threads)
+    one_rate, n_one, el_one = rate(1)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"value": all_rate, "unit": "edge-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n_all} steps at {threads} threads ({el_all:.1f} s) and {n_one} steps at 1 thread "
+                      f"({el_one:.1f} s) of the same graph and inputs (FP32 C/OpenMP port of "
+                      f"leviathan_kernel.cu:93-264,391-438)",
+            "value_1_thread": one_rate, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "cpu_model": _cpu_model()}
+
+
+def ws100k_record(steps=10_000, warmup=100):
+    """Watts-Strogatz N=100,000 (k=20, p=0.1, delays U{1..50}), plasticity and the gain
+    controller on: the plain FP32 drop-in (leviathan_init's configuration) over 10,000 steps in
+    100-step graph chunks, and its windowed r against the FP64 oracle's 10,000 steps of the same
+    inputs (the committed fixture tests/golden/ws100k_fp64.npz, when present)."""
+    from leviathan_v1_amd import LeviathanEngine
+
+    wl = make_workload("ws100k")
+    eng = LeviathanEngine(*wl.engine_args())
+    try:
+        eng.step_n(wl.dt, warmup)
+        eng.close()
+        eng = LeviathanEngine(*wl.engine_args())  # the measured run starts from the inputs
+        t0 = time.perf_counter()
+        r = eng.step_n(wl.dt, steps)
+        el = time.perf_counter() - t0
+        prof = eng.profile_steps(wl.dt, 50)
+    finally:
+        eng.close()
+    out = {"workload": wl.name, "N": wl.N, "E": wl.E, "form": "plain FP32 drop-in (direct/8)", "steps": steps,
+           "ms_per_step": 1e3 * el / steps, "edge_steps_per_s": wl.E * steps / el,
+           "hbm_gbs": algorithmic_bytes(wl.N, wl.E, "f32") * steps / el / 1e9,
+           "step_kernel_ms": prof["step_kernel_ms"], "finalize_kernel_ms": prof["finalize_kernel_ms"],
+           "note": "the 41 MB working set fits the 256 MB Infinity Cache: algorithmic GB/s, not HBM-bound"}
+    gold = os.path.join(ROOT, "tests", "golden", "ws100k_fp64.npz")
+    if os.path.exists(gold) and steps == 10_000:
+        ref = np.load(gold)["r"]
+        win = lambda x: np.asarray(x, np.float64)[1000:].reshape(-1, 500)
+        out["windowed_r_vs_fp64"] = {"window": 500, "warmup": 1000,
+                                     "max_abs_mean_diff": float(np.abs(win(r).mean(1) - win(ref).mean(1)).max()),
+                                     "max_abs_var_diff": float(np.abs(win(r).var(1) - win(ref).var(1)).max())}
+    return out
 
 
 def copy_calibration(nbytes: int, reps: int = 3) -> None:
@@ -208,6 +272,8 @@ def main():
     ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="skip the sub-records (WS N=10^5 drop-in, 256-network ensemble) of the default line")
     ap.add_argument("--forms", default="f32,f32-tuned,u16",
                     help="f32: plain drop-in (direct/8); f32-tuned, u16: --schedule (default autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -264,6 +330,17 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
         log("cpu", json.dumps(cpu))
+    subs = {}
+    if args.workload == "ba10m" and not args.no_sub and args.n is None:
+        subs["ws100k_plain_fp32"] = ws100k_record()
+        log("ws100k", json.dumps(subs["ws100k_plain_fp32"]))
+        from leviathan_v1_amd import ensemble_bench
+
+        ens = ensemble_bench.run(steps=200, warmup=20, seq_steps=5)
+        subs["ensemble256"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "steps", "config",
+                                                   "sequential_plain", "speedup_vs_sequential", "step_kernel_ms",
+                                                   "r_of_K")}
+        log("ensemble", ens["value"], ens["speedup_vs_sequential"])
     out = {
         "metric": METRIC,
         "value": head["edge_steps_per_s"],
@@ -294,6 +371,7 @@ def main():
                                              head.get("scatter_steps")),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
+        **subs,
     }
     print(json.dumps(out))
 

@@ -273,6 +273,14 @@ class COracle(_StateMixin):
         return float(self.run(dt, 1, None if self.drive_b is None else np.array([u]))[0])
 
 
+def set_threads(n: int) -> int:
+    """Threads of the OpenMP restatements from now on; returns the previous count."""
+    lib = _load()
+    prev = int(lib.lev_oracle_max_threads())
+    lib.lev_oracle_set_threads(ctypes.c_int(int(n)))
+    return prev
+
+
 def circ_err(a, b):
     """|a - b| on the circle, in radians."""
     d = np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)

@@ -44,6 +44,13 @@ typedef struct {
   int32_t gain_controller;
 } lev_oracle_params;
 
+#include <omp.h>
+
+/* Threads the OpenMP restatements use from the next call on (the CPU baseline
+ * is timed at one thread and at all of them). */
+void lev_oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int lev_oracle_max_threads(void) { return omp_get_max_threads(); }
+
 #define PREFETCH 0
 
 /* FP64 instantiation */

@@ -66,28 +66,59 @@ def make_ws100k(steps=10_000):
                         r=r, g_final=np.array(o.g))
 
 
-def make_ba10m(steps=2_000):
+def make_ba10m(steps=2_000, resume_dir=None):
     """Barabasi-Albert N=10^7, m=10, delays U{1..50}, plasticity and the gain controller on:
     r(t) and g(t) of 2,000 FP64 steps (the OpenMP FP64 oracle, whose phases and weights are
-    bit-identical to the sequential one).  Written in chunks, so an interrupted run resumes."""
+    bit-identical to the sequential one).  The oracle's whole state is saved every 100 steps
+    under resume_dir (default: the repository's _work/ folder), so an interrupted run resumes
+    from the last save with identical results."""
     wl = workloads.scale_skew(0)
     dg = digest(wl)
     print(f"ba10m: N={wl.N} E={wl.E} digest {dg[:16]}", flush=True)
     o = oracle.COracle(*wl.engine_args(), dtype=np.float64, parallel=True)
+    resume_dir = resume_dir or os.path.join(ROOT, "_work", "ba10m_resume")
+    os.makedirs(resume_dir, exist_ok=True)
+    arrays = ("w", "theta", "theta_hat", "omega_hat", "buf", "r", "g")
+    r = np.zeros(steps)
+    g = np.zeros(steps)
+    done = 0
+    meta = os.path.join(resume_dir, "meta.npz")
+    if os.path.exists(meta):
+        with np.load(meta) as z:
+            if str(z["digest"]) == dg:
+                done = int(z["done"])
+                o.buf_idx = int(z["buf_idx"])
+                o.g = float(z["g_now"])
+                for name in arrays:
+                    a = np.load(os.path.join(resume_dir, name + ".npy"), mmap_mode="r")
+                    if name == "r":
+                        r[:] = a
+                    elif name == "g":
+                        g[:] = a
+                    else:
+                        getattr(o, name)[...] = a
+                o.step_count = done
+                print(f"ba10m: resumed at step {done}", flush=True)
     t0 = time.time()
-    r = np.empty(steps)
-    g = np.empty(steps)
     chunk = 50
-    for s in range(0, steps, chunk):
+    for s in range(done, steps, chunk):
         r[s:s + chunk] = o.run(DT, chunk)
         g[s + chunk - 1] = o.g
         print(f"ba10m: {s + chunk} steps, {time.time() - t0:.0f} s, r={r[s + chunk - 1]:.6f} g={o.g:.6f}",
               flush=True)
+        if (s + chunk) % 100 == 0 and s + chunk < steps:
+            for name in arrays:
+                src = r if name == "r" else g if name == "g" else getattr(o, name)
+                np.save(os.path.join(resume_dir, name + ".tmp.npy"), src)
+                os.replace(os.path.join(resume_dir, name + ".tmp.npy"), os.path.join(resume_dir, name + ".npy"))
+            np.savez(meta + ".tmp.npz", digest=np.array(dg), done=np.array(s + chunk), buf_idx=np.array(o.buf_idx),
+                     g_now=np.array(float(o.g)))
+            os.replace(meta + ".tmp.npz", meta)
     g_mask = np.zeros(steps, bool)
     g_mask[chunk - 1::chunk] = True
     np.savez_compressed(os.path.join(HERE, "ba10m_fp64.npz"), digest=np.array(dg), dt=np.array(DT), r=r,
                         g_steps=np.nonzero(g_mask)[0], g=g[g_mask], g_final=np.array(o.g),
-                        theta_mean_cos=np.array(np.cos(o.theta).mean()), seconds=np.array(time.time() - t0))
+                        theta_mean_cos=np.array(np.cos(o.theta).mean()))
 
 
 if __name__ == "__main__":

@@ -10,14 +10,15 @@ def say(*msg):
     print("[case]", *msg, flush=True)
 
 
-def oracle_from_engine(eng, wl, params=None, dtype=np.float64, drive_b=None):
+def oracle_from_engine(eng, wl, params=None, dtype=np.float64, drive_b=None, parallel=False):
     """An oracle holding exactly the engine's current state (phases, predictor,
     weights, every history slot, gain), so one oracle step from it isolates the
-    error of one GPU step."""
+    error of one GPU step.  parallel: the OpenMP FP64 restatement (same phases and
+    weights as the sequential one; for the 10^7-node graph)."""
     theta = eng.get_theta()
     thh, omh = eng.get_predictor()
     o = COracle(wl.N, wl.max_delay, wl.graph.row_ptr, wl.graph.col_idx, wl.delays, eng.get_weights(), theta, thh,
-                wl.omega, params=params or Params(), dtype=dtype)
+                wl.omega, params=params or Params(), dtype=dtype, parallel=parallel)
     o.omega_hat[:] = omh
     B = o.B
     for d in range(wl.max_delay + 1):
