@@ -258,6 +258,20 @@ int leviathan_features_get(LeviathanHandle* handle, int64_t row0, int64_t nrows,
 int leviathan_features_colsum(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* weights,
                               double* out);
 
+/* leviathan_features_colsum without any device allocation: no counterpart in
+ * the reference.  For callers that cannot call hipMalloc (serving loops,
+ * harnesses that forbid it, the readout's own refinement loop, which calls the
+ * moments every iteration).  The staged weights and the output live in a
+ * file-scope device array of this library (one per GPU, 576 KiB, held until
+ * the process ends) instead of per-call allocations; the partial sums use the
+ * handle's buffer from leviathan_features_setup.  Limits: 2 * n_nodes <= 8192
+ * features, and nrows <= 65536 when weights are given; larger calls fail with
+ * an error naming the limit.  Same results as leviathan_features_colsum (the
+ * same kernels).  Every call on one GPU shares its array: calls that overlap
+ * on one GPU (two threads, two handles) race; calls one after another do not. */
+int leviathan_features_colsum_noalloc(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* weights,
+                                      double* out);
+
 /* out[t] = sum_f (X[row0 + t][f] - mean[f]) * w[f], FP64 accumulation. */
 int leviathan_features_matvec(LeviathanHandle* handle, int64_t row0, int64_t nrows, const double* mean,
                               const double* w, double* out);

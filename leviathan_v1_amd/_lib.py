@@ -92,6 +92,7 @@ SIGNATURES = {
     "leviathan_features_rows": (c_int64, [vp]),
     "leviathan_features_get": (c_int, [vp, c_int64, c_int64, vp]),
     "leviathan_features_colsum": (c_int, [vp, c_int64, c_int64, vp, vp]),
+    "leviathan_features_colsum_noalloc": (c_int, [vp, c_int64, c_int64, vp, vp]),
     "leviathan_features_matvec": (c_int, [vp, c_int64, c_int64, vp, vp, vp]),
     "leviathan_features_gram": (c_int, [vp, c_int64, c_int64, vp, c_int, vp, vp]),
     "leviathan_features_factor": (c_int, [vp, c_int64, c_int64, vp, c_int, c_double, vp, vp]),

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,39 @@ void set_err(const char* fmt, ...) {
       return false;                                                                        \
     }                                                                                      \
   } while (0)
+
+// Device scratch of leviathan_features_colsum_noalloc: the staged row weights
+// (kNoAllocRows doubles) followed by the output (kNoAllocFeatures doubles).
+// Each GPU holds its own copy until the process ends.  Host code never passes
+// the array's name to a kernel or a copy (on the host it is only a stand-in):
+// it uses the address hipGetSymbolAddress returns per device (scratch_ptr).
+constexpr int64_t kNoAllocRows = 65536;
+constexpr int32_t kNoAllocFeatures = 8192;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocRows + kNoAllocFeatures];
+
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Looks up (once per device, with that device current) and returns the device
+// address of this GPU's scratch array, or null after recording the error.
+double* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    set_err("device %d is beyond the %d devices the allocation-free scratch table holds", device, kMaxDevices);
+    return nullptr;
+  }
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_err("hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[device] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[device];
+}
 
 struct GraphEntry {
   uint32_t dt_bits;
@@ -643,6 +677,9 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     return true;
   };
   if (!hip_ok(hipGetDevice(&h->device), "hipGetDevice")) return fail();
+  // the allocation-free readout entry point's scratch address on this device
+  // (looked up once per device; a failure here is reported by that entry point)
+  if (!scratch_ptr(h->device)) g_last_error.clear();
   if (!hip_ok(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return fail();
 
   const size_t hsize = cfg.history == LEV_HISTORY_F32 ? 4 : 2;
@@ -1764,6 +1801,37 @@ int leviathan_features_colsum(LeviathanHandle* h, int64_t row0, int64_t nrows, c
   if (d_out) (void)hipFree(d_out);
   if (e != hipSuccess) {
     set_err("features_colsum: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int leviathan_features_colsum_noalloc(LeviathanHandle* h, int64_t row0, int64_t nrows, const double* weights,
+                                      double* out) {
+  if (!check_handle(h) || !feat_range(h, row0, nrows)) return -1;
+  if (!out) {
+    set_err("null output array");
+    return -1;
+  }
+  const int32_t F = 2 * h->feat_n;
+  if (F > kNoAllocFeatures || (weights && nrows > kNoAllocRows)) {
+    set_err("features_colsum_noalloc: %d features and %lld weighted rows exceed the allocation-free maximum "
+            "(%d features, %lld weighted rows); use leviathan_features_colsum",
+            F, (long long)(weights ? nrows : 0), kNoAllocFeatures, (long long)kNoAllocRows);
+    return -1;
+  }
+  double* base = scratch_ptr(h->device);
+  if (!base) return -1;
+  double* d_w = weights ? base : nullptr;
+  double* d_out = base + kNoAllocRows;
+  hipError_t e = hipSuccess;
+  if (weights) e = hipMemcpyAsync(d_w, weights, sizeof(double) * nrows, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = lev::launch_colsum(h->d_X, F, row0, nrows, d_w, F, h->d_cs_partial, lev::kColsumChunks, d_out, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * F, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_err("features_colsum_noalloc: %s", hipGetErrorString(e));
     return -1;
   }
   return 0;
