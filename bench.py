@@ -270,6 +270,8 @@ def main():
     ap.add_argument("--scatter-steps", default="auto", choices=["auto", "1", "2", "4"],
                     help="steps per scatter pass of the two-phase schedules (default: timed and chosen)")
     ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
+    ap.add_argument("--sweep-alphas", default="", help="reservoir: comma-separated ridge parameters to refit with")
+    ap.add_argument("--sweep-washouts", default="", help="reservoir: comma-separated washout lengths to refit with")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sub", action="store_true",
@@ -294,7 +296,9 @@ def main():
         from leviathan_v1_amd import reservoir_bench
 
         out = reservoir_bench.run(steps=max(args.steps, 2000), static=args.static_reservoir,
-                                  graph_chunk=args.graph_chunk)
+                                  graph_chunk=args.graph_chunk,
+                                  sweep_alphas=[float(v) for v in args.sweep_alphas.split(",") if v],
+                                  sweep_washouts=[int(v) for v in args.sweep_washouts.split(",") if v])
         print(json.dumps({"metric": "reservoir readout NRMSE (u(t+10)), BF16 MFMA Gram + FP64 refinement",
                           "value": out["nrmse_test"], "unit": "NRMSE", "n_gpus": 1, "steps": max(args.steps, 2000),
                           "higher_is_better": False, "data": "synthetic", **out}))

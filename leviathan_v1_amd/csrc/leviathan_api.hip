@@ -330,6 +330,7 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.drive_b = h->drive_on ? h->d_drive_b : nullptr;
   a.drive_u = h->d_drive_u;
   a.hbuf = pl.kind == LEV_SCHEDULE_STAGED ? h->d_hbuf2 : h->d_hbuf;
+  a.hstride = 1;
   a.hpos = h->d_hpos;
   a.lpos = pl.d_lpos;
   a.dt = dt;
@@ -376,7 +377,6 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   sa.xcd_remap = 1;
   sa.blocks = h->d_sources;
   sa.hbuf = h->d_hbuf;
-  sa.hbuf_stride = h->E;
   sa.steps = 1;
   sa.step = h->d_step;
   return sa;
@@ -399,9 +399,12 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
   if (pl.kind == LEV_SCHEDULE_DIRECT)
     return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
   const bool staged = pl.kind == LEV_SCHEDULE_STAGED;
-  const size_t step_bytes = (size_t)h->E * (h->cfg.history == LEV_HISTORY_F32 ? 4 : 2);
-  // the array step k's row pass reads: bucketed values (binned) or regrouped ones (staged)
-  void* hk = static_cast<char*>(staged ? h->d_hbuf2 : h->d_hbuf) + (size_t)k * step_bytes;
+  const size_t hsize = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
+  // the values step k's row pass reads: the regrouped array of step k (staged), or component k of
+  // the bucketed slots, which hold the K steps' values side by side (binned)
+  void* hk = staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)k * h->E * hsize)
+                    : static_cast<void*>(static_cast<char*>(h->d_hbuf) + (size_t)k * hsize);
+  const int32_t hstride = staged ? 1 : K;
   hipError_t e = hipSuccess;
   if (k == 0) {
     lev::ScatterArgs sa = scatter_args(h, pl);
@@ -417,6 +420,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     ea.ring_pitch = h->ring_pitch;
     ea.n_slots = h->n_slots;
     ea.hbuf = hk;
+    ea.hstride = hstride;
     ea.step = h->d_step;
     e = lev::launch_early(h->cfg.history, h->cfg.layout, ea, h->stream);
   }
@@ -437,6 +441,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
   if (e == hipSuccess) {
     StepArgs a = step_args(h, dt, pl);
     a.hbuf = hk;
+    a.hstride = hstride;
     e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, staged ? 2 : 1, a, h->stream);
   }
   return e;

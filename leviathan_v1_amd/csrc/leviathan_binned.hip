@@ -48,19 +48,21 @@ __device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d,
 }
 
 // K = number of consecutive steps whose values one pass gathers (1, 2 or 4):
-// step t + k's values go to hbuf + k * hbuf_stride.  Values of edges with
-// d < k are read from slots this pass must not trust (not yet written for
-// step t + k); early_kernel overwrites them before that step's row pass.
+// slot p of hbuf holds the K values of its edge side by side (HVec), step
+// t + k's in component k.  Values of edges with d < k are read from slots
+// this pass must not trust (not yet written for step t + k); early_kernel
+// overwrites them before that step's row pass.
 template <class H, int LAYOUT, int K>
 __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   // a source block holds up to CAP out-edges, EPT per lane.  (With K = 4 a
   // 512-lane workgroup, 8 edges per lane at 92 instead of 164 VGPRs, took
   // 1.92 ms per pass at BA N = 10^7, u16, against 1.82 ms with 256 lanes.)
+  using V = HVec<H, K>;
   constexpr int T = kBlock;
   constexpr int CAP = kBlock * kScatterEpt;
   constexpr int EPT = CAP / T;
   __shared__ uint8_t s_esrc[CAP];
-  __shared__ H s_val[K > 1 ? 2 : 1][CAP + 1];  // the block's values in slot order (two buffers when K > 1)
+  __shared__ V s_val[CAP + 1];  // the block's value vectors in slot order
   __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   const int32_t S = a.n_slots;
   const int32_t cur = (int32_t)(step % S);
   const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
-  H* __restrict__ hbuf = reinterpret_cast<H*>(a.hbuf);
+  V* __restrict__ hbuf = reinterpret_cast<V*>(a.hbuf);
   const int64_t pitch = a.ring_pitch;
 
   if (!(blk.w & kBlockLongRow)) {
@@ -142,60 +144,54 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
       }
     };
-    H v[EPT], vn[EPT];
-    load_values(v, 0);
+    // the K steps' values of every edge, all loads in flight at once, packed per edge
+    V pv[EPT];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      // the next step's values are in flight across this step's barrier
-      if (q + 1 < K) load_values(vn, q + 1);
-      H* sv = s_val[q & 1];
-      // place in slot order; a lane past the block's edges writes the spare
-      // slot CAP, so that no load is sunk into a branch and waited for alone
+      H v[EPT];
+      load_values(v, q);
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
+    }
+    // place in slot order; a lane past the block's edges writes the spare
+    // slot CAP, so that no load is sunk into a branch and waited for alone
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int32_t le = k * T + tid;
+      s_val[le < nnz ? (rk[k] & 0xffff) : CAP] = pv[k];
+    }
+#pragma unroll
+    for (int j = 0; j < kScatterRuns / T; ++j) {
+      const int32_t r = j * T + tid;
+      if (r < nr) s_base[r] = tb[j];
+    }
+    __syncthreads();
+    // write in slot order: the values bound for one bucket leave as one contiguous run.
+    // The branch on the run count is uniform over the block: with a per-lane
+    // choice between s_base and run_base the compiler read both through one
+    // flat load of a selected pointer, and every store then waited for all
+    // earlier loads and stores (vmcnt(0)).
+    // All slots and values are read from LDS first, into registers of their
+    // own, and the stores follow: with one register set reused per store,
+    // each store's source registers were guarded by a wait for every
+    // earlier store (vmcnt(0)).
+    if (nr_all <= kScatterRuns) {
+      int32_t at[EPT];
+      V val[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
+        at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
+        val[k] = s_val[le];
+      }
+#pragma unroll
+      for (int k = 0; k < EPT; ++k)
+        if (k * T + tid < nnz) hbuf[at[k]] = val[k];
+    } else {
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
-        sv[le < nnz ? (rk[k] & 0xffff) : CAP] = v[k];
-      }
-      if (q == 0) {
-#pragma unroll
-        for (int j = 0; j < kScatterRuns / T; ++j) {
-          const int32_t r = j * T + tid;
-          if (r < nr) s_base[r] = tb[j];
-        }
-      }
-      __syncthreads();
-      // write in slot order: the values bound for one bucket leave as one contiguous run.
-      // The branch on the run count is uniform over the block: with a per-lane
-      // choice between s_base and run_base the compiler read both through one
-      // flat load of a selected pointer, and every store then waited for all
-      // earlier loads and stores (vmcnt(0)).
-      // All slots and values are read from LDS first, into registers of their
-      // own, and the stores follow: with one register set reused per store,
-      // each store's source registers were guarded by a wait for every
-      // earlier store (vmcnt(0)).
-      H* out = hbuf + (int64_t)q * a.hbuf_stride;
-      if (nr_all <= kScatterRuns) {
-        int32_t at[EPT];
-        H val[EPT];
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-          const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
-          at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
-          val[k] = sv[le];
-        }
-#pragma unroll
-        for (int k = 0; k < EPT; ++k)
-          if (k * T + tid < nnz) out[at[k]] = val[k];
-      } else {
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-          const int32_t le = k * T + tid;
-          if (le < nnz) out[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = sv[le];
-        }
-      }
-      if (q + 1 < K) {
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) v[k] = vn[k];
+        if (le < nnz) hbuf[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = s_val[le];
       }
     }
   } else {
@@ -211,18 +207,16 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         d[k] = __builtin_nontemporal_load(a.delay + c);
         p[k] = __builtin_nontemporal_load(a.pos + c);
       }
+      V v[4];
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
-        H v[4];
+      for (int q = 0; q < K; ++q)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          v[k] = ring[ring_index<LAYOUT>(pitch, s0, ahead_slot(cur, q, d[k], S))];  // a hub's line: reused, cached
-        H* out = hbuf + (int64_t)q * a.hbuf_stride;
+          v[k].x[q] = ring[ring_index<LAYOUT>(pitch, s0, ahead_slot(cur, q, d[k], S))];  // a hub's line: reused, cached
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int32_t le = base + k * T + tid;
-          if (le < nnz) out[p[k]] = v[k];
-        }
+      for (int k = 0; k < 4; ++k) {
+        const int32_t le = base + k * T + tid;
+        if (le < nnz) hbuf[p[k]] = v[k];
       }
     }
   }
@@ -241,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void early_kernel(EarlyArgs a) {
   int32_t slot = cur - (int32_t)a.delay[i];
   slot += (slot < 0) ? S : 0;
   const H* ring = reinterpret_cast<const H*>(a.ring);
-  reinterpret_cast<H*>(a.hbuf)[a.pos[i]] = ring[ring_index<LAYOUT>(a.ring_pitch, a.src[i], slot)];
+  reinterpret_cast<H*>(a.hbuf)[(int64_t)a.pos[i] * a.hstride] = ring[ring_index<LAYOUT>(a.ring_pitch, a.src[i], slot)];
 }
 
 // Regroup pass of the staged schedule: kRegroupChunksPerWg consecutive chunks
@@ -338,20 +332,27 @@ __global__ __launch_bounds__(kBlock) void regroup_kernel(RegroupArgs a) {
 }
 
 // Regroup of the values of K consecutive steps at once (several steps per
-// scatter pass): one chunk per workgroup.  The rank, the run heads' scan and
-// each position's staged slot are shared by the K steps; the steps' values are
-// all loaded first, then placed and written in turn through one LDS buffer
-// (two barriers per step; a second buffer would cost a sixth of the
-// workgroups per CU, and the pass was slower with it).
+// scatter pass): one chunk per workgroup.  Each bucket slot holds its edge's
+// K values side by side, so one load, one LDS placement and one LDS read move
+// all K; the rank, the run heads' scan and each position's staged slot are
+// computed once, and the K values leave to the K steps' staged arrays.
 template <class H, int K>
 __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
+  using V = HVec<H, K>;
   constexpr int PER = kRegroupChunk / kBlock;
-  __shared__ H s_val[kRegroupChunk];
+  // The values are placed in LDS one step at a time through a buffer of
+  // single values, two barriers per step.  (Placing whole vectors, one LDS
+  // write and read per position, needs 32 KB per workgroup at two uint16
+  // steps: 4 instead of 6 workgroups per CU, and the pass took 0.71 instead of
+  // 0.61 ms at BA N = 10^7.)
+  constexpr bool kWhole = false;
+  __shared__ V s_vec[kWhole ? kRegroupChunk : 1];
+  __shared__ H s_one[kWhole ? 1 : kRegroupChunk];
   __shared__ uint16_t s_run[kRegroupChunk];  // run heads, then each position's run
   __shared__ int2 s_runs[kRegroupRuns];
   __shared__ int32_t s_wsum[kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const H* __restrict__ src = reinterpret_cast<const H*>(a.src);
+  const V* __restrict__ src = reinterpret_cast<const V*>(a.src);
   H* __restrict__ dst = reinterpret_cast<H*>(a.dst);
   // each XCD takes a contiguous run of chunks: the few chunks that fill one
   // staged line (consecutive source ranges of a bucket) meet in one L2.
@@ -362,15 +363,14 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   const int32_t n = ch.y, r0 = ch.z, nr = ch.w;
 
   // unconditional loads: a lane past the chunk reads element 0 and discards it
-  H v[K][PER];
+  V v[PER];
   int32_t rk[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int32_t l = k * kBlock + tid;
     const int64_t c = l < n ? (int64_t)ch.x + l : 0;
     rk[k] = a.rank[c];
-#pragma unroll
-    for (int q = 0; q < K; ++q) v[q][k] = src[q * a.stride + c];
+    v[k] = src[c];
   }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -383,7 +383,12 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int32_t l = k * kBlock + tid;
-    if (l < n) s_val[rk[k]] = v[0][k];
+    if (l < n) {
+      if constexpr (kWhole)
+        s_vec[rk[k]] = v[k];
+      else
+        s_one[rk[k]] = v[k].x[0];
+    }
   }
   __syncthreads();
 
@@ -414,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   }
   __syncthreads();
 
-  // each position's staged slot, shared by the K steps
+  // each position's staged slot, then the K steps' stores
   int32_t off[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -425,22 +430,34 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
       off[k] = rr.y + (l - rr.x);
     }
   }
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    if (q > 0) {
-      __syncthreads();  // the previous step's values have left the LDS
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int32_t l = k * kBlock + tid;
-        if (l < n) s_val[rk[k]] = v[q][k];
-      }
-      __syncthreads();
-    }
-    H* out = dst + q * a.stride;
+  if constexpr (kWhole) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int32_t l = k * kBlock + tid;
-      if (l < n) out[off[k]] = s_val[l];
+      if (l < n) {
+        const V val = s_vec[l];
+#pragma unroll
+        for (int q = 0; q < K; ++q) dst[q * a.stride + off[k]] = val.x[q];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      if (q > 0) {
+        __syncthreads();  // the previous step's values have left the LDS
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int32_t l = k * kBlock + tid;
+          if (l < n) s_one[rk[k]] = v[k].x[q];
+        }
+        __syncthreads();
+      }
+      H* out = dst + q * a.stride;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int32_t l = k * kBlock + tid;
+        if (l < n) out[off[k]] = s_one[l];
+      }
     }
   }
 }

@@ -67,6 +67,7 @@ struct StepArgs {
   const float* drive_u;   // [kDriveSteps] ring, u_t at t % kDriveSteps
   const void* hbuf;       // binned schedule: delayed phases written by the scatter kernel
   const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
+  int32_t hstride;        // binned schedule: elements per hbuf slot (the K interleaved steps of a scatter pass)
   const uint16_t* lpos;   // staged schedule: [E] position of each CSR edge's value in its block's range
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
@@ -105,10 +106,10 @@ struct RegroupArgs {
   const int4* chunks;        // {first slot, count, first run, run count}
   const uint16_t* rank;      // [E] bucket slot -> index in its chunk's staged order
   const int2* runs;          // {first index in the chunk's staged order, staged slot}
-  const void* src;           // bucketed values
-  void* dst;                 // staged values
+  const void* src;           // bucketed values, [E][steps] (a vector of the pass's steps per slot)
+  void* dst;                 // staged values, [steps][stride]
   int32_t num_chunks;
-  int64_t stride;            // elements between consecutive steps' arrays (steps > 1)
+  int64_t stride;            // elements between consecutive steps' staged arrays (steps > 1)
   int32_t steps;             // steps whose values one pass regroups (1, 2 or kMaxScatterSteps)
 };
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
@@ -131,8 +132,8 @@ struct ScatterArgs {
   int32_t num_blocks;
   int32_t xcd_remap;
   const int4* blocks;       // pack_item layout: {first source, first out-edge, end out-edge, sources << 1 | long}
-  void* hbuf;               // [steps][hbuf_stride] delayed phases, bucketed
-  int64_t hbuf_stride;      // elements between the value arrays of consecutive steps
+  void* hbuf;               // [E][steps] delayed phases, bucketed: each slot holds the values of the
+                            // pass's steps side by side (one vector per edge)
   int32_t steps;            // 1, 2 or kMaxScatterSteps steps gathered in one pass
   const int64_t* step;
 };
@@ -152,7 +153,16 @@ struct EarlyArgs {
   int64_t ring_pitch;
   int32_t n_slots;
   void* hbuf;               // the values the step's row pass reads (bucketed, or regrouped when staged)
+  int32_t hstride;          // elements per slot of that array (binned: the pass's interleaved steps)
   const int64_t* step;
+};
+
+// The delayed values of a scatter pass's K steps for one edge, side by side
+// (4 or 8 bytes for uint16 history): one LDS access and one store per edge
+// and pass instead of one per step.
+template <class H, int K>
+struct alignas(sizeof(H) * K) HVec {
+  H x[K];
 };
 
 // Launchers (leviathan_kernels.hip).  Each returns the launch status.

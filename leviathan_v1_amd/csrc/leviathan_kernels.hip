@@ -184,7 +184,8 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           slot += (slot < 0) ? S : 0;
           hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
         } else {
-          hr[k] = hbuf[(uint32_t)jj[k]];
+          // the bucketed array keeps a scatter pass's steps side by side: hstride values per slot
+          hr[k] = hbuf[(int64_t)(uint32_t)jj[k] * a.hstride];
         }
       }
 #pragma unroll
@@ -279,6 +280,8 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           int32_t slot = cur - dd[k];
           slot += (slot < 0) ? S : 0;
           hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
+        } else if constexpr (GATHER == 1) {
+          hr[k] = hbuf[(int64_t)(uint32_t)jj[k] * a.hstride];
         } else {
           hr[k] = hbuf[(uint32_t)jj[k]];
         }

@@ -14,7 +14,8 @@ namespace lev {
 // 2*pi rounded to FP32 (the reference's 2.0f * CUDART_PI_F, leviathan_kernel.cu:102).
 constexpr float kTwoPi = 6.28318548f;
 // uint16 exact-wrap angles: q in [0, 65536) stands for q * 2*pi / 65536 radians.
-constexpr float kU16ToRad = 9.58737992e-05f;  // 2*pi / 65536
+constexpr float kU16ToRad = 0x1.921fb6p-14f;      // 2*pi / 65536 rounded to FP32
+constexpr float kU16ToRadTail = -0x1.777a5cp-39f; // 2*pi / 65536 - kU16ToRad, rounded to FP32
 constexpr float kRadToU16 = 10430.3784f;      // 65536 / (2*pi)
 
 // Exactly the reference's per-node wraps (true division): prep's [-pi, pi)
@@ -37,7 +38,15 @@ struct HistCodec<float> {
 
 template <>
 struct HistCodec<uint16_t> {
-  __device__ __forceinline__ static float decode(uint16_t q) { return (float)q * kU16ToRad; }
+  // The angle q * 2*pi/65536 rounded once to FP32: q times the constant split
+  // into an FP32 head (kU16ToRad) and tail, added by one fused multiply-add
+  // (correctly rounded for all 65,536 codes).  The product with the head alone
+  // is 2.8e-8 relative too large on average (the head's rounding), a bias that
+  // adds up over a hub's ten thousand incoming edges.
+  __device__ __forceinline__ static float decode(uint16_t q) {
+    const float qf = (float)q;
+    return fmaf(qf, kU16ToRad, qf * kU16ToRadTail);
+  }
   // Round to the nearest 2*pi/65536 step and wrap modulo 65536 (so 2*pi -> 0).
   __device__ __forceinline__ static uint16_t encode(float x) {
     return (uint16_t)((uint32_t)__float2int_rn(x * kRadToU16) & 0xFFFFu);

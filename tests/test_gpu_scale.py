@@ -5,8 +5,9 @@ up to ~10^4), delays uniform in [1, 50], FP32 and uint16 history.
     taken from the engine's exported state (the bar of the other GPU tests), and the
     step's r within 1e-6 of the FP64 order parameter of the phases it wrote;
   * the benchmark's form (uint16 history, staged schedule with 8 edges per lane, two steps
-    per scatter pass): one two-step block against two FP64 oracle steps from the exported
-    state, and bit-identical to the same two steps taken one scatter pass each;
+    per scatter pass): each step of a two-step block against an FP64 oracle step from the
+    engine's state, and the block bit-identical to the same two steps taken one scatter pass
+    each;
   * 2,000 steps of the uint16 form and of the plain FP32 drop-in against the FP64 oracle's
     2,000 steps of the same inputs (the committed fixture tests/golden/ba10m_fp64.npz,
     written by tests/golden/make_golden.py with the OpenMP FP64 oracle): windowed
@@ -72,30 +73,37 @@ def test_ba10m_one_step_vs_fp64_oracle(ba10m, history, schedule):
 
 
 def test_ba10m_two_step_block_vs_fp64_oracle(ba10m):
-    """The benchmark's form takes its steps in blocks of two: one scatter pass and one regroup
-    pass gather the delayed phases of both steps, then two row passes run.  From the state at
-    a block boundary, the block's two steps are compared with two FP64 oracle steps.  The
-    oracle's second step reads the delay-1 phases as the engine stores them (the uint16
-    encoding of the first step's phases, exported from an engine that took that step alone);
-    every other value comes from the oracle's own FP64 arithmetic.  The block is also
-    bit-identical to the same two steps taken one scatter pass each."""
+    """The benchmark's form takes its steps in blocks: one scatter pass and one regroup pass
+    gather the delayed phases of two steps, then two row passes run.  From the state at a block
+    boundary, each step of the block is compared with an FP64 oracle step taken from the
+    engine's own state before it (1e-5 relative): the first from the block engine's exported
+    state, the second from the state of an engine that took the first step alone, which is
+    bit-identical to the block's (the block's end state is checked bit for bit against it).
+    Two free-running FP64 steps are not the bar: the heaviest rows (~15,000 incoming edges,
+    coupling derivative ~350) multiply any difference after the first step by ~27 in the
+    second, so that comparison measures the dynamics' sensitivity, not the kernel.  It is
+    printed for reference."""
     say("ba10m two-step block, u16 staged/8, 2 steps per pass")
     a = make(ba10m, history="u16", schedule="staged", edges_per_thread=8)
     b = make(ba10m, history="u16", schedule="staged", edges_per_thread=8)
     a.set_scatter_steps(2)
     assert (a.scatter_steps, b.scatter_steps) == (2, 1)
     np.testing.assert_array_equal(a.step_n(DT, 20), b.step_n(DT, 20))
-    o = oracle_from_engine(a, ba10m, parallel=True)
-    o.step(DT)
-    b.step(DT)  # the first step alone: its ring slot is what the block's second step reads
-    o.buf[:, o.buf_idx] = b.get_history(0)
-    o.step(DT)
+    o1 = oracle_from_engine(a, ba10m, parallel=True)
+    o1.step(DT)
+    b.step(DT)  # the block's first step, alone
+    e1t, e1w = theta_rel_err(b.get_theta(), o1.theta), w_rel_err(b.get_weights(), o1.w)
+    o2 = oracle_from_engine(b, ba10m, parallel=True)
+    o2.step(DT)
+    o1.step(DT)  # free-running second FP64 step, for reference
     b.step(DT)
     ra = a.step_n(DT, 2)  # one block: scatter + regroup of two steps, two row passes
     th = a.get_theta()
-    et, ew = theta_rel_err(th, o.theta), w_rel_err(a.get_weights(), o.w)
-    say("two-step theta rel err", et, "w rel err", ew, "r", float(ra[-1]))
-    assert et < TOL and ew < TOL
+    e2t, e2w = theta_rel_err(th, o2.theta), w_rel_err(a.get_weights(), o2.w)
+    say("step 1: theta", e1t, "w", e1w, "| step 2 from the engine's state: theta", e2t, "w", e2w,
+        "| free-running FP64 two steps: theta", theta_rel_err(th, o1.theta), "| r", float(ra[-1]))
+    assert e1t < TOL and e1w < TOL
+    assert e2t < TOL and e2w < TOL
     assert np.array_equal(th, b.get_theta())
     assert np.array_equal(a.get_weights(), b.get_weights())
     for d in (0, 1, 2, 50):
@@ -104,15 +112,15 @@ def test_ba10m_two_step_block_vs_fp64_oracle(ba10m):
     b.close()
 
 
-@pytest.mark.parametrize("form", ["u16_staged8_k2", "f32_plain"])
+@pytest.mark.parametrize("form", ["u16_staged8_k2", "u16_staged8_k4", "f32_plain"])
 def test_ba10m_windowed_r_vs_fp64_fixture(ba10m, golden10m, form):
     """2,000 steps against the FP64 oracle's 2,000 steps of the same inputs.  The dynamics are
     chaotic, so trajectories separate; the bar is on windowed (200-step) mean r and r
     variance, within 5e-3 in every window."""
     say("ba10m 2000 steps vs FP64 fixture", form)
-    if form == "u16_staged8_k2":
+    if form.startswith("u16"):
         eng = make(ba10m, history="u16", schedule="staged", edges_per_thread=8)
-        eng.set_scatter_steps(2)
+        eng.set_scatter_steps(int(form[-1]))
     else:
         eng = make(ba10m)  # the plain FP32 drop-in, as leviathan_init configures it
     r = eng.step_n(DT, 2000)

@@ -43,18 +43,28 @@ def test_ensemble_256_networks_5000_steps():
     tail = r[-1000:].mean(axis=0)
     say("r(K) over the last 1,000 steps, every 32nd network:", np.round(tail[::32], 3).tolist())
     ens.close()
-    # the three smallest networks against their FP64 oracle runs (same gains, controller off)
+    # The three smallest networks (N ~ 2,000) against their FP64 oracle runs (same gains,
+    # controller off).  The dynamics are chaotic and at this size r fluctuates by ~1/sqrt(N)
+    # ~ 0.02 on slow time scales, so two FP64 trajectories whose initial phases differ by the
+    # uint16 history's quantisation (up to pi/65536 per node) also separate in windowed r.  The
+    # bar is 5e-3 or twice that FP64 chaos floor (measured for each network), whichever is
+    # larger.
     for k in np.argsort(spec.sizes)[:3]:
-        o = oracle.COracle(*wls[k].engine_args(), params=oracle.Params(g0=float(spec.gains[k]),
-                                                                         gain_controller=False),
-                           dtype=np.float64, parallel=True)
-        ref = o.run(DT, 5000)
+        prm = oracle.Params(g0=float(spec.gains[k]), gain_controller=False)
+        args = wls[k].engine_args()
+        ref = oracle.COracle(*args, params=prm, dtype=np.float64, parallel=True).run(DT, 5000)
+        rng = np.random.default_rng(int(k))
+        th_p = np.asarray(args[6], np.float64) + rng.uniform(-np.pi / 65536, np.pi / 65536, wls[k].N)
+        alt = oracle.COracle(*args[:6], th_p, th_p, args[8], params=prm, dtype=np.float64, parallel=True).run(DT, 5000)
         m, v = windowed(r[:, k], 1000, 1000)
         mr, vr = windowed(ref, 1000, 1000)
-        say("net", int(k), "N", wls[k].N, "K", float(spec.gains[k]), "max |mean r diff|",
-            float(np.abs(m - mr).max()), "max |var diff|", float(np.abs(v - vr).max()))
-        assert np.abs(m - mr).max() < 5e-3, k
-        assert np.abs(v - vr).max() < 5e-3, k
+        ma, va = windowed(alt, 1000, 1000)
+        floor_m, floor_v = float(np.abs(ma - mr).max()), float(np.abs(va - vr).max())
+        dm, dv = float(np.abs(m - mr).max()), float(np.abs(v - vr).max())
+        say("net", int(k), "N", wls[k].N, "K", float(spec.gains[k]), "max |mean r diff|", dm, "max |var diff|", dv,
+            "FP64 chaos floor", floor_m, floor_v)
+        assert dm < max(5e-3, 2 * floor_m), k
+        assert dv < max(5e-3, 2 * floor_v), k
 
 
 @pytest.fixture(scope="module")
