@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       om = a.omega[i];
       if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
-      s_theta[tid] = rs_.th;
+      s_theta[tid] = sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th;  // uint16: revolutions (edge_update_rev)
       s_gate[tid] = rs_.gate;
       rs = a.row_ptr[i] - e0;
       re = a.row_ptr[i + 1] - e0;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * kBlock + tid;
-        if (le < nnz) s_contrib[le] = HistCodec<H>::decode(sv[k]);
+        if (le < nnz) s_contrib[le] = sizeof(H) == 2 ? (float)sv[k] : HistCodec<H>::decode(sv[k]);  // uint16: the code
       }
     }
     __syncthreads();
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         }
       }
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) hh[k] = HistCodec<H>::decode(hr[k]);
+      for (int k = 0; k < EPT; ++k) hh[k] = sizeof(H) == 2 ? (float)hr[k] : HistCodec<H>::decode(hr[k]);
     }
     if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
     // one base address for the lane's weight stores (each store adds a constant
@@ -201,7 +201,11 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       if (le < nnz) {
         const int lr = s_erow[le];
         float c;
-        const float wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        float wn;
+        if constexpr (sizeof(H) == 2)
+          wn = edge_update_rev((uint16_t)hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        else
+          wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
         s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
         wlane[k * kBlock] = wn;
       }
@@ -253,6 +257,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
     }
     __syncthreads();
     const float th_i = s_theta[0];
+    const float th_rev = th_i * kInvTwoPi;
     const float gate = s_gate[0];
     CompSum acc;
     for (int32_t base = 0; base < nnz; base += kBlock * 4) {
@@ -291,8 +296,11 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         const int32_t le = base + k * kBlock + tid;
         if (le < nnz) {
           float c;
-          const float wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min,
-                                       a.w_max, &c);
+          float wn;
+          if constexpr (sizeof(H) == 2)
+            wn = edge_update_rev((uint16_t)hr[k], th_rev, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+          else
+            wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
           acc.add(c);
           a.w[e0 + le] = wn;
         }

@@ -160,6 +160,27 @@ __device__ __forceinline__ float edge_update(float h, float th_i, float gate, fl
   return wv < w_min ? w_min : wv;
 }
 
+// uint16 history: the delayed phase is exactly q / 65536 revolutions, so the
+// edge takes sin and cos of delta = h - theta_i in revolutions (th_rev is
+// theta_i / (2*pi), formed once per row) with the hardware v_sin_f32 and
+// v_cos_f32, which take revolutions: three instructions where the bounded
+// polynomial routine takes about thirty.  Measured at the 10^7-node graph:
+// one step within 2.2e-7 (theta) and 2.8e-7 (W) of the FP64 oracle, and
+// 2,000 steps' windowed r within 3.9e-3 of the FP64 run (3.7e-3 with the
+// polynomial routine, whose error the uint16 storage error dwarfs).
+__device__ __forceinline__ float edge_update_rev(uint16_t q, float th_rev, float gate, float w, float dt, float eta,
+                                                 float lam, float w_min, float w_max, float* contrib) {
+  const float d = fmaf((float)q, 0x1p-16f, -th_rev);
+  const float sn = __builtin_amdgcn_sinf(d);
+  const float cs = __builtin_amdgcn_cosf(d);
+  *contrib = w * sn;
+  float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
+  if (!__builtin_isfinite(wv)) wv = w_min;
+  wv = wv > w_max ? w_max : wv;
+  return wv < w_min ? w_min : wv;
+}
+constexpr float kInvTwoPi = 0.159154943f;  // 1 / (2*pi)
+
 // Logical row-block index for a workgroup.  Workgroups are dealt to the 8
 // XCDs round-robin; this mapping gives each XCD one contiguous run of row
 // blocks so neighbouring rows (which share source nodes) meet in one L2.
