@@ -614,6 +614,10 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     set_err("edges_per_thread must be 4, 8 or 16 (got %d)", cfg.edges_per_thread);
     return nullptr;
   }
+  if (!(cfg.w_min <= cfg.w_max)) {  // the step clamps weights into [w_min, w_max]
+    set_err("w_min (%g) must not exceed w_max (%g)", (double)cfg.w_min, (double)cfg.w_max);
+    return nullptr;
+  }
   if (cfg.graph_chunk < 0 || cfg.graph_chunk > lev::kRHistSteps) {
     set_err("graph_chunk must be in [0, %d]", lev::kRHistSteps);
     return nullptr;

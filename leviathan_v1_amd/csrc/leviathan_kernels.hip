@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * kBlock + tid;
-        if (le < nnz) hh[k] = s_contrib[jj[k]];
+        hh[k] = le < nnz ? s_contrib[jj[k]] : 0.0f;
       }
     } else {
       // every lane's column, delay or slot is a valid one (edge 0 for a lane past the edges)
@@ -192,22 +192,28 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       for (int k = 0; k < EPT; ++k) hh[k] = sizeof(H) == 2 ? (float)hr[k] : HistCodec<H>::decode(hr[k]);
     }
     if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
-    // one base address for the lane's weight stores (each store adds a constant
-    // offset), so that no store recomputes a 64-bit address
-    float* const wlane = a.w + e0 + tid;
+    // the lane's weight stores address the weights by a 32-bit element index
+    // from the kernel argument's base (E < 2^31), so no store needs 64-bit
+    // address arithmetic of its own
+    const uint32_t wlane = (uint32_t)(e0 + tid);
+    // The edge arithmetic runs on every lane (a lane past the block's edges
+    // works on row 0 and a defined value and stores nothing), so that only the
+    // two stores are predicated and the constants stay in registers across the
+    // unrolled edges.
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * kBlock + tid;
-      if (le < nnz) {
-        const int lr = s_erow[le];
-        float c;
-        float wn;
-        if constexpr (sizeof(H) == 2)
-          wn = edge_update_rev((uint16_t)hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
-        else
-          wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+      const int32_t le = k * kBlock + tid;  // < CAP: the LDS read is in range
+      const bool live = le < nnz;
+      const int lr = live ? (int)s_erow[le] : 0;
+      float c;
+      float wn;
+      if constexpr (sizeof(H) == 2)
+        wn = edge_update_rev(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+      else
+        wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+      if (live) {
         s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
-        wlane[k * kBlock] = wn;
+        a.w[wlane + k * kBlock] = wn;
       }
     }
     __syncthreads();
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
           float c;
           float wn;
           if constexpr (sizeof(H) == 2)
-            wn = edge_update_rev((uint16_t)hr[k], th_rev, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+            wn = edge_update_rev((float)hr[k], th_rev, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
           else
             wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
           acc.add(c);

@@ -140,6 +140,16 @@ __device__ __forceinline__ float row_integrate(const RowState& s, float om, floa
   return x;
 }
 
+// The reference's guard and clamp of a new weight (leviathan_kernel.cu:159-163):
+// NaN and infinities to w_min, then clamp to [w_min, w_max] (w_min <= w_max is
+// checked when a handle is created).  The clamp is one v_med3_f32 of a finite
+// value (fminf and fmaxf made the compiler quiet both bounds for every edge,
+// and compares with selects took four instructions).
+__device__ __forceinline__ float clamp_weight(float wv, float w_min, float w_max) {
+  if (!__builtin_isfinite(wv)) wv = w_min;
+  return __builtin_amdgcn_fmed3f(wv, w_min, w_max);
+}
+
 // Hebbian update of one edge (leviathan_kernel.cu:157-163); returns the new
 // weight and writes the coupling contribution w_old * sin(delta).  The
 // reference wraps delta to [-pi, pi) first (:146); sin and cos are
@@ -153,11 +163,7 @@ __device__ __forceinline__ float edge_update(float h, float th_i, float gate, fl
   sincos_fast(diff, &sn, &cs);
   *contrib = w * sn;
   float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
-  // NaN and infinities to w_min, then the clamp as compares and selects (fminf
-  // and fmaxf quieted both bounds again for every edge)
-  if (!__builtin_isfinite(wv)) wv = w_min;
-  wv = wv > w_max ? w_max : wv;
-  return wv < w_min ? w_min : wv;
+  return clamp_weight(wv, w_min, w_max);
 }
 
 // uint16 history: the delayed phase is exactly q / 65536 revolutions, so the
@@ -168,16 +174,15 @@ __device__ __forceinline__ float edge_update(float h, float th_i, float gate, fl
 // one step within 2.2e-7 (theta) and 2.8e-7 (W) of the FP64 oracle, and
 // 2,000 steps' windowed r within 3.9e-3 of the FP64 run (3.7e-3 with the
 // polynomial routine, whose error the uint16 storage error dwarfs).
-__device__ __forceinline__ float edge_update_rev(uint16_t q, float th_rev, float gate, float w, float dt, float eta,
+// q is the uint16 code as a float (exact).
+__device__ __forceinline__ float edge_update_rev(float q, float th_rev, float gate, float w, float dt, float eta,
                                                  float lam, float w_min, float w_max, float* contrib) {
-  const float d = fmaf((float)q, 0x1p-16f, -th_rev);
+  const float d = fmaf(q, 0x1p-16f, -th_rev);
   const float sn = __builtin_amdgcn_sinf(d);
   const float cs = __builtin_amdgcn_cosf(d);
   *contrib = w * sn;
-  float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
-  if (!__builtin_isfinite(wv)) wv = w_min;
-  wv = wv > w_max ? w_max : wv;
-  return wv < w_min ? w_min : wv;
+  const float wv = fmaf(dt, fmaf(eta * cs, gate, -lam * w), w);
+  return clamp_weight(wv, w_min, w_max);
 }
 constexpr float kInvTwoPi = 0.159154943f;  // 1 / (2*pi)
 

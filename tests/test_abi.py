@@ -120,3 +120,12 @@ def test_null_handle_calls_report_error():
     assert np.isnan(lib.leviathan_step(None, 0.05))
     assert "null" in _lib.last_error()
     assert lib.leviathan_step_n(None, 0.05, 3, None) == -1
+
+
+def test_init_rejects_inverted_weight_bounds():
+    """The step clamps every weight into [w_min, w_max]; a configuration with w_min > w_max is
+    refused when the handle is created (before any device work)."""
+    from leviathan_v1_amd import LeviathanEngine
+
+    with pytest.raises(RuntimeError, match="w_min"):
+        LeviathanEngine(*_good(), params=dict(w_min=1.0, w_max=0.5))
