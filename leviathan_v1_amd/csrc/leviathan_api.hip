@@ -133,6 +133,7 @@ struct LeviathanHandle {
   int32_t max_delay = 0;
   int32_t n_slots = 0;      // S = max_delay + 2
   int64_t ring_pitch = 0;
+  size_t ring_bytes = 0;    // the history ring's size
   int32_t n_nets = 1;
   int64_t step = 0;         // host mirror of the device step counter
   size_t vram = 0;
@@ -372,6 +373,7 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   sa.run_base = pl.d_run_base;
   sa.ring = h->d_ring;
   sa.ring_pitch = h->ring_pitch;
+  sa.ring_bytes = (int64_t)h->ring_bytes;
   sa.n_slots = h->n_slots;
   sa.num_blocks = h->n_sources;
   sa.xcd_remap = 1;
@@ -701,6 +703,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   }
   const size_t ring_elems = cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)N * h->ring_pitch
                                                                : (size_t)h->n_slots * h->ring_pitch;
+  h->ring_bytes = ring_elems * hsize;
 
   // work decomposition: the configured direct plan always, the other direct
   // plans and the binned plan when a schedule other than DIRECT is asked for
@@ -1476,15 +1479,11 @@ struct StateSnapshot {
   // Takes the snapshot and detaches the hub statistics (trial steps record nothing).
   bool take(LeviathanHandle* hh) {
     h = hh;
-    const size_t hist_bytes = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-    const size_t ring_bytes = (h->cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)h->N * h->ring_pitch
-                                                                        : (size_t)h->n_slots * h->ring_pitch) *
-                              hist_bytes;
     bufs = {{h->d_w, sizeof(float) * (size_t)h->E, nullptr},
             {h->d_theta, sizeof(float) * (size_t)h->N, nullptr},
             {h->d_theta_hat, sizeof(float) * (size_t)h->N, nullptr},
             {h->d_omega_hat, sizeof(float) * (size_t)h->N, nullptr},
-            {h->d_ring, ring_bytes, nullptr},
+            {h->d_ring, h->ring_bytes, nullptr},
             {h->d_ctrl, sizeof(NetCtrl) * (size_t)h->n_nets, nullptr},
             {h->d_step, sizeof(int64_t), nullptr}};
     step = h->step;

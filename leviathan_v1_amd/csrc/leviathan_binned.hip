@@ -52,7 +52,13 @@ __device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d,
 // t + k's in component k.  Values of edges with d < k are read from slots
 // this pass must not trust (not yet written for step t + k); early_kernel
 // overwrites them before that step's row pass.
-template <class H, int LAYOUT, int K>
+// OFF32 (node-major rings below 2^32 bytes): each history read is a 32-bit
+// byte offset from the ring's base (the scalar-base form of the load), one
+// multiply per edge and then one slot step per further step with one wrap
+// test, in place of a 64-bit multiply per read.  At BA N = 10^7, uint16, four
+// steps per pass: node-major 2.40 -> 2.20 ms per pass; slot-major got slower
+// with it (1.61 -> 1.70 ms) and keeps the 64-bit index.
+template <class H, int LAYOUT, int K, bool OFF32>
 __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   // a source block holds up to CAP out-edges, EPT per lane.  (With K = 4 a
   // 512-lane workgroup, 8 edges per lane at 92 instead of 164 VGPRs, took
@@ -129,29 +135,58 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // in (delay, source) order, so the lanes of one load read one slot row for
     // consecutive sources; for q > 0 those rows were read for q - 1 by the
     // neighbouring delay and are in cache.
-    auto load_values = [&](H (&v)[EPT], int q) {
+    // the K steps' values of every edge, all loads in flight at once, packed per edge
+    V pv[EPT];
+    if constexpr (OFF32) {
+      static_assert(LAYOUT == 0, "32-bit offsets are used for node-major rings");
+      // slot of step t + q is slot0 + q, less S once it passes the ring's end
+      const char* base = reinterpret_cast<const char*>(a.ring);
+      uint32_t off0[EPT];
+      int32_t slot0[EPT];
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
-        const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
-        int32_t node;
-        if constexpr (LAYOUT == 0) {
-          node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-        } else {
-          node = s0 + (dd[k] >> 8);
-        }
-        const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
-        v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+        const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+        slot0[k] = ahead_slot(cur, 0, dd[k] & 255, S);
+        off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
       }
-    };
-    // the K steps' values of every edge, all loads in flight at once, packed per edge
-    V pv[EPT];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-      H v[EPT];
-      load_values(v, q);
+      for (int q = 0; q < K; ++q) {
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
+        for (int k = 0; k < EPT; ++k) {
+          const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? (uint32_t)(S * sizeof(H)) : 0u);
+          pv[k].x[q] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
+        }
+      }
+    } else {
+      // One step's loads go into registers of their own and are packed
+      // afterwards.  (Small changes of this code's shape moved the pass a lot:
+      // loading straight into the vectors made the compiler issue the uint16
+      // history loads one at a time, 2.2 instead of 1.6 ms per four-step
+      // pass, and a lambda that returned the source node took the FP32
+      // two-step pass from 1.33 to 1.51 ms, both at BA N = 10^7.)
+      auto load_values = [&](H (&v)[EPT], int q) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;
+          const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
+          int32_t node;
+          if constexpr (LAYOUT == 0) {
+            node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+          } else {
+            node = s0 + (dd[k] >> 8);
+          }
+          const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
+          v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+        }
+      };
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        H v[EPT];
+        load_values(v, q);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
+      }
     }
     // place in slot order; a lane past the block's edges writes the spare
     // slot CAP, so that no load is sunk into a branch and waited for alone
@@ -489,20 +524,23 @@ hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <class H, int K>
+static void scatter_launch(int layout, const ScatterArgs& a, hipStream_t s) {
+  const dim3 grid(a.num_blocks), block(kBlock);
+  if (layout != 0)
+    scatter_kernel<H, 1, K, false><<<grid, block, 0, s>>>(a);
+  else if (a.ring_bytes < (int64_t(1) << 32))
+    scatter_kernel<H, 0, K, true><<<grid, block, 0, s>>>(a);
+  else
+    scatter_kernel<H, 0, K, false><<<grid, block, 0, s>>>(a);
+}
+
 template <int K>
 static void scatter_launch(int history, int layout, const ScatterArgs& a, hipStream_t s) {
-  const dim3 grid(a.num_blocks), block(kBlock);
-  if (history == 0) {
-    if (layout == 0)
-      scatter_kernel<float, 0, K><<<grid, block, 0, s>>>(a);
-    else
-      scatter_kernel<float, 1, K><<<grid, block, 0, s>>>(a);
-  } else {
-    if (layout == 0)
-      scatter_kernel<uint16_t, 0, K><<<grid, block, 0, s>>>(a);
-    else
-      scatter_kernel<uint16_t, 1, K><<<grid, block, 0, s>>>(a);
-  }
+  if (history == 0)
+    scatter_launch<float, K>(layout, a, s);
+  else
+    scatter_launch<uint16_t, K>(layout, a, s);
 }
 
 hipError_t launch_scatter(int history, int layout, const ScatterArgs& a, hipStream_t s) {

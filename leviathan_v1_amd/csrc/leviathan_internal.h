@@ -128,6 +128,7 @@ struct ScatterArgs {
   const uint16_t* dsrc;     // [E] slot-major ring: each edge's source index in its block << 8 | delay
   const void* ring;
   int64_t ring_pitch;
+  int64_t ring_bytes;       // size of the ring (a node-major ring below 2^32 bytes is read through 32-bit offsets)
   int32_t n_slots;
   int32_t num_blocks;
   int32_t xcd_remap;
