@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   constexpr int T = kBlock;
   constexpr int CAP = kBlock * kScatterEpt;
   constexpr int EPT = CAP / T;
-  __shared__ uint8_t s_esrc[CAP];
+  __shared__ uint8_t s_esrc[LAYOUT == 0 ? CAP : 1];  // node-major: each edge's source in the block
   __shared__ V s_val[CAP + 1];  // the block's value vectors in slot order
   __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
@@ -121,11 +121,13 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * T + tid;
-      const int32_t c = le < nnz ? c0 + le : 0;
+      // unsigned 32-bit element indices: the loads take the scalar-base form
+      const uint32_t c = (uint32_t)(le < nnz ? c0 + le : 0);
       if constexpr (LAYOUT == 0) {
         dd[k] = __builtin_nontemporal_load(a.delay + c);
       } else {
-        dd[k] = le < nnz ? (int32_t)__builtin_nontemporal_load(a.dsrc + c) : 0;
+        const int32_t code = __builtin_nontemporal_load(a.dsrc + c);
+        dd[k] = le < nnz ? code : 0;
       }
       rk[k] = __builtin_nontemporal_load(a.rank_run + c);
     }
@@ -138,25 +140,42 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // the K steps' values of every edge, all loads in flight at once, packed per edge
     V pv[EPT];
     if constexpr (OFF32) {
-      static_assert(LAYOUT == 0, "32-bit offsets are used for node-major rings");
-      // slot of step t + q is slot0 + q, less S once it passes the ring's end
+      // slot of step t + q is slot0 + q, less S once it passes the ring's end;
+      // one slot further is one element (node-major) or one slot row (slot-major)
       const char* base = reinterpret_cast<const char*>(a.ring);
+      const uint32_t slot_bytes = LAYOUT == 0 ? (uint32_t)sizeof(H) : (uint32_t)pitch * (uint32_t)sizeof(H);
+      const uint32_t wrap_bytes = (uint32_t)S * slot_bytes;  // a whole turn of the ring
       uint32_t off0[EPT];
       int32_t slot0[EPT];
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
-        const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-        slot0[k] = ahead_slot(cur, 0, dd[k] & 255, S);
-        off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
+        const int32_t d = dd[k] & 255;
+        slot0[k] = cur - d + (cur < d ? S : 0);  // slot of step t: -S < cur - d < S
+        if constexpr (LAYOUT == 0) {
+          const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+          off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
+        } else {
+          const int32_t node = s0 + (dd[k] >> 8);
+          off0[k] = (uint32_t)slot0[k] * slot_bytes + (uint32_t)node * (uint32_t)sizeof(H);
+        }
       }
 #pragma unroll
       for (int q = 0; q < K; ++q) {
+        H v[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-          const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? (uint32_t)(S * sizeof(H)) : 0u);
-          pv[k].x[q] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
+          const uint32_t o = off0[k] + (uint32_t)q * slot_bytes;
+          uint32_t off;
+          if constexpr (LAYOUT == 1)
+            off = q == 0 ? o : min(o, o - wrap_bytes);  // slot-major: past the ring's end, o less a turn is the smaller
+          else
+            off = o - (slot0[k] >= S - q ? wrap_bytes : 0u);
+          const H* src = reinterpret_cast<const H*>(base + off);
+          v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
         }
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
       }
     } else {
       // One step's loads go into registers of their own and are packed
@@ -527,9 +546,13 @@ hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
 template <class H, int K>
 static void scatter_launch(int layout, const ScatterArgs& a, hipStream_t s) {
   const dim3 grid(a.num_blocks), block(kBlock);
-  if (layout != 0)
+  // 32-bit offsets: every offset the kernel forms, up to K - 1 slots past the ring's end, is below 2^32
+  const bool off32 = a.ring_bytes + (int64_t)K * a.ring_pitch * (int64_t)sizeof(H) < (int64_t(1) << 32);
+  if (layout != 0 && off32)
+    scatter_kernel<H, 1, K, true><<<grid, block, 0, s>>>(a);
+  else if (layout != 0)
     scatter_kernel<H, 1, K, false><<<grid, block, 0, s>>>(a);
-  else if (a.ring_bytes < (int64_t(1) << 32))
+  else if (off32)
     scatter_kernel<H, 0, K, true><<<grid, block, 0, s>>>(a);
   else
     scatter_kernel<H, 0, K, false><<<grid, block, 0, s>>>(a);
