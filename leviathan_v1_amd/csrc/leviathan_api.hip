@@ -647,8 +647,10 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     return nullptr;
   }
   const size_t hist_bytes = cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  // Default bucket: 1 MB of delayed phases, a quarter of one XCD's L2.
-  if (cfg.bucket_edges == 0) cfg.bucket_edges = (int)((1 << 20) / hist_bytes);
+  // Default bucket: 2^18 incoming edges (1 MB of FP32 delayed phases per step).  Measured at BA
+  // N = 10^7, uint16, staged/8, four steps per pass, ms per step: 2^17 edges 1.251, 2^18 1.174,
+  // 3 * 2^17 1.178, 2^19 1.207 (smaller buckets: shorter scatter runs; larger: longer regroups).
+  if (cfg.bucket_edges == 0) cfg.bucket_edges = 1 << 18;
   if (cfg.schedule != LEV_SCHEDULE_DIRECT && cfg.bucket_edges < 1024) {
     set_err("bucket_edges must be at least 1024");
     return nullptr;
@@ -721,7 +723,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   }
   const size_t ring_elems = cfg.layout == LEV_LAYOUT_NODE_MAJOR ? (size_t)N * h->ring_pitch
                                                                : (size_t)h->n_slots * h->ring_pitch;
-  h->ring_bytes = ring_elems * hsize;
+  h->ring_bytes = ring_elems * hsize;  // (allocated with kScatterWinCols elements more: the scatter's window
+                                       // of the last slot row reads past the ring's end)
 
   // work decomposition: the configured direct plan always, the other direct
   // plans and the binned plan when a schedule other than DIRECT is asked for
@@ -832,7 +835,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       !dalloc_edges(h, &h->d_delay, (size_t)h->E) || !dalloc_edges(h, &h->d_w, (size_t)h->E) ||
       !dalloc(h, &h->d_theta, (size_t)N) || !dalloc(h, &h->d_theta_hat, (size_t)N) ||
       !dalloc(h, &h->d_omega, (size_t)N) || !dalloc(h, &h->d_omega_hat, (size_t)N) ||
-      !dalloc(h, (char**)&h->d_ring, ring_elems * hsize) || !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
+      !dalloc(h, (char**)&h->d_ring, (ring_elems + lev::kScatterWinCols) * hsize) || !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
       !dalloc(h, &h->d_chunks, (size_t)(h->n_partials + lev::kChunkPartials - 1) / lev::kChunkPartials) ||
       !dalloc(h, &h->d_fin_done, 0) ||
       !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||

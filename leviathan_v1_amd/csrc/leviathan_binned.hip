@@ -47,18 +47,27 @@ __device__ __forceinline__ int32_t ahead_slot(int32_t cur, int32_t k, int32_t d,
   return slot;
 }
 
+// How a short source block reads the history ring (MODE of scatter_kernel):
+// 64-bit element indices, 32-bit byte offsets from the ring's base, or (slot-
+// major rings of at most scatter_window_slots slots) a window of the ring in LDS.
+constexpr int kScatterGather64 = 0, kScatterOff32 = 1, kScatterWindow = 2;
+
 // K = number of consecutive steps whose values one pass gathers (1, 2 or 4):
 // slot p of hbuf holds the K values of its edge side by side (HVec), step
 // t + k's in component k.  Values of edges with d < k are read from slots
 // this pass must not trust (not yet written for step t + k); early_kernel
 // overwrites them before that step's row pass.
-// OFF32 (node-major rings below 2^32 bytes): each history read is a 32-bit
-// byte offset from the ring's base (the scalar-base form of the load), one
-// multiply per edge and then one slot step per further step with one wrap
+// kScatterOff32 (node-major rings below 2^32 bytes): each history read is a
+// 32-bit byte offset from the ring's base (the scalar-base form of the load),
+// one multiply per edge and then one slot step per further step with one wrap
 // test, in place of a 64-bit multiply per read.  At BA N = 10^7, uint16, four
-// steps per pass: node-major 2.40 -> 2.20 ms per pass; slot-major got slower
-// with it (1.61 -> 1.70 ms) and keeps the 64-bit index.
-template <class H, int LAYOUT, int K, bool OFF32>
+// steps per pass: node-major 2.40 -> 2.20 ms per pass.
+// kScatterWindow (slot-major rings): the block's sources' window of the whole
+// ring is loaded into LDS with coalesced 16-byte loads that depend only on the
+// block descriptor, and the edges read their values there.  At BA N = 10^7,
+// uint16, four steps per pass: 0.401 -> 0.332 ms per step for the scatter and
+// its early gathers (the gathers issued 64 dependent 2-byte loads per lane).
+template <class H, int LAYOUT, int K, int MODE>
 __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   // a source block holds up to CAP out-edges, EPT per lane.  (With K = 4 a
   // 512-lane workgroup, 8 edges per lane at 92 instead of 164 VGPRs, took
@@ -67,8 +76,16 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   constexpr int T = kBlock;
   constexpr int CAP = kBlock * kScatterEpt;
   constexpr int EPT = CAP / T;
+  static_assert(MODE != kScatterWindow || LAYOUT == 1, "the window form reads a slot-major ring");
+  static_assert(MODE != kScatterOff32 || LAYOUT == 0, "32-bit offsets are used for node-major rings");
+  constexpr int WSLOTS = scatter_window_slots(sizeof(H));
+  constexpr size_t kValBytes = sizeof(V) * (CAP + 1);
+  constexpr size_t kWinBytes = MODE == kScatterWindow ? (size_t)WSLOTS * kScatterWinCols * sizeof(H) : 0;
   __shared__ uint8_t s_esrc[LAYOUT == 0 ? CAP : 1];  // node-major: each edge's source in the block
-  __shared__ V s_val[CAP + 1];  // the block's value vectors in slot order
+  // the block's value vectors in slot order; before them, in the window form,
+  // the ring's window of the block's sources
+  __shared__ __attribute__((aligned(16))) char s_raw[kValBytes > kWinBytes ? kValBytes : kWinBytes];
+  V* const s_val = reinterpret_cast<V*>(s_raw);
   __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
   const int32_t b = xcd_block(blockIdx.x, a.num_blocks, a.xcd_remap);
@@ -139,12 +156,10 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
     // neighbouring delay and are in cache.
     // the K steps' values of every edge, all loads in flight at once, packed per edge
     V pv[EPT];
-    if constexpr (OFF32) {
-      // slot of step t + q is slot0 + q, less S once it passes the ring's end;
-      // one slot further is one element (node-major) or one slot row (slot-major)
+    if constexpr (MODE == kScatterOff32) {
+      // slot of step t + q is slot0 + q, less S once it passes the ring's end
       const char* base = reinterpret_cast<const char*>(a.ring);
-      const uint32_t slot_bytes = LAYOUT == 0 ? (uint32_t)sizeof(H) : (uint32_t)pitch * (uint32_t)sizeof(H);
-      const uint32_t wrap_bytes = (uint32_t)S * slot_bytes;  // a whole turn of the ring
+      const uint32_t wrap_bytes = (uint32_t)S * (uint32_t)sizeof(H);  // a whole turn of the ring
       uint32_t off0[EPT];
       int32_t slot0[EPT];
 #pragma unroll
@@ -152,31 +167,56 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         const int32_t le = k * T + tid;
         const int32_t d = dd[k] & 255;
         slot0[k] = cur - d + (cur < d ? S : 0);  // slot of step t: -S < cur - d < S
-        if constexpr (LAYOUT == 0) {
-          const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-          off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
-        } else {
-          const int32_t node = s0 + (dd[k] >> 8);
-          off0[k] = (uint32_t)slot0[k] * slot_bytes + (uint32_t)node * (uint32_t)sizeof(H);
-        }
+        const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+        off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
       }
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         H v[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-          const uint32_t o = off0[k] + (uint32_t)q * slot_bytes;
-          uint32_t off;
-          if constexpr (LAYOUT == 1)
-            off = q == 0 ? o : min(o, o - wrap_bytes);  // slot-major: past the ring's end, o less a turn is the smaller
-          else
-            off = o - (slot0[k] >= S - q ? wrap_bytes : 0u);
-          const H* src = reinterpret_cast<const H*>(base + off);
-          v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+          const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? wrap_bytes : 0u);
+          v[k] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
         }
 #pragma unroll
         for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
       }
+    } else if constexpr (MODE == kScatterWindow) {
+      // The block's sources [s0, s0 + nsrc) in every slot row: an 8-aligned
+      // window of kScatterWinCols columns per slot comes in with coalesced
+      // 16-byte loads, which wait for nothing but the block's descriptor (the
+      // gathers wait for the edge codes first), and each edge reads its K
+      // values from LDS.  The ring holds kScatterWinCols elements of padding
+      // past its end for the last slot row's window.
+      H* const s_win = reinterpret_cast<H*>(s_raw);
+      constexpr int EPC = 16 / (int)sizeof(H);          // elements per 16-byte chunk
+      constexpr int CPR = kScatterWinCols / EPC;        // chunks per window row
+      constexpr int NCH = (WSLOTS * CPR + T - 1) / T;   // chunk loads per lane at most
+      const int32_t w0 = s0 & ~7;
+      const int32_t nch = S * CPR;
+      uint4 ch[NCH];
+#pragma unroll
+      for (int j = 0; j < NCH; ++j) {
+        const int32_t id = j * T + tid < nch ? j * T + tid : 0;
+        const int32_t r = id / CPR, c = id % CPR;
+        ch[j] = *reinterpret_cast<const uint4*>(ring + (int64_t)r * pitch + w0 + c * EPC);
+      }
+#pragma unroll
+      for (int j = 0; j < NCH; ++j) {
+        const int32_t id = j * T + tid;
+        if (id < nch) *reinterpret_cast<uint4*>(s_win + (id / CPR) * kScatterWinCols + (id % CPR) * EPC) = ch[j];
+      }
+      __syncthreads();
+      const int32_t col0 = s0 - w0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
+          pv[k].x[q] = s_win[slot * kScatterWinCols + col0 + (dd[k] >> 8)];
+        }
+      }
+      __syncthreads();  // every window value is read before the placement below overwrites them
     } else {
       // One step's loads go into registers of their own and are packed
       // afterwards.  (Small changes of this code's shape moved the pass a lot:
@@ -546,16 +586,17 @@ hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
 template <class H, int K>
 static void scatter_launch(int layout, const ScatterArgs& a, hipStream_t s) {
   const dim3 grid(a.num_blocks), block(kBlock);
-  // 32-bit offsets: every offset the kernel forms, up to K - 1 slots past the ring's end, is below 2^32
-  const bool off32 = a.ring_bytes + (int64_t)K * a.ring_pitch * (int64_t)sizeof(H) < (int64_t(1) << 32);
-  if (layout != 0 && off32)
-    scatter_kernel<H, 1, K, true><<<grid, block, 0, s>>>(a);
+  const bool off32 = a.ring_bytes < (int64_t(1) << 32);
+  // (the 32-bit offset gathers were slower than the 64-bit ones on a slot-major ring: BA N = 10^7, FP32,
+  // two steps per pass, 0.77 against 0.66 ms per step)
+  if (layout != 0 && a.n_slots <= scatter_window_slots(sizeof(H)))
+    scatter_kernel<H, 1, K, kScatterWindow><<<grid, block, 0, s>>>(a);
   else if (layout != 0)
-    scatter_kernel<H, 1, K, false><<<grid, block, 0, s>>>(a);
+    scatter_kernel<H, 1, K, kScatterGather64><<<grid, block, 0, s>>>(a);
   else if (off32)
-    scatter_kernel<H, 0, K, true><<<grid, block, 0, s>>>(a);
+    scatter_kernel<H, 0, K, kScatterOff32><<<grid, block, 0, s>>>(a);
   else
-    scatter_kernel<H, 0, K, false><<<grid, block, 0, s>>>(a);
+    scatter_kernel<H, 0, K, kScatterGather64><<<grid, block, 0, s>>>(a);
 }
 
 template <int K>
