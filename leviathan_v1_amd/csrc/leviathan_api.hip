@@ -245,8 +245,10 @@ size_t stage_stride(const LeviathanHandle* h) { return ((size_t)h->E + lev::kEdg
 
 // The scatter's per-edge rank and per-slot-order-index run, packed rank | run << 16 (both are
 // indexed by the block's first out-edge + the lane's index), and its run tables.
+// With the scatter's window form (hsc non-null) the per-edge words are its
+// codes (window_scatter_codes) instead of rank | run.
 bool upload_runs(LeviathanHandle* h, const std::vector<uint16_t>& rank, const lev::HostScatterRuns& r,
-                 uint32_t** rank_run, int32_t** ptr, int32_t** base) {
+                 uint32_t** rank_run, int32_t** ptr, int32_t** base, const lev::HostScatter* hsc = nullptr) {
   if (rank.size() != r.run_idx.size()) {
     set_err("scatter ranks and runs differ in length");
     return false;
@@ -254,7 +256,10 @@ bool upload_runs(LeviathanHandle* h, const std::vector<uint16_t>& rank, const le
   if (!dalloc(h, rank_run, rank.size()) || !dalloc(h, ptr, r.run_ptr.size()) || !dalloc(h, base, r.run_base.size()))
     return false;
   std::vector<uint32_t> packed(rank.size());
-  for (size_t c = 0; c < rank.size(); ++c) packed[c] = (uint32_t)rank[c] | (uint32_t)r.run_idx[c] << 16;
+  if (hsc)
+    lev::window_scatter_codes(*hsc, rank, r, packed);
+  else
+    for (size_t c = 0; c < rank.size(); ++c) packed[c] = (uint32_t)rank[c] | (uint32_t)r.run_idx[c] << 16;
   hipError_t e = hipSuccess;
   if (!packed.empty())
     e = hipMemcpy(*rank_run, packed.data(), sizeof(uint32_t) * packed.size(), hipMemcpyHostToDevice);
@@ -754,6 +759,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   std::vector<uint8_t> early_delay;
   std::vector<uint32_t> early_csr, bin_early_pos, st_early_pos[3];
   const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
+  // the scatter kernel's launcher makes the same choice (leviathan_binned.hip)
+  const bool scatter_window = lev::scatter_uses_window(cfg.layout, h->n_slots, hsize);
   const bool want_staged = cfg.schedule == LEV_SCHEDULE_STAGED || cfg.schedule == LEV_SCHEDULE_AUTO;
   if (want_binned || want_staged) {
     if (!lev::build_scatter_plan(N, row_ptr, col_idx, delays, hsc, perr)) {
@@ -882,7 +889,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
                 "copy sources") &&
          hip_ok(hipMemcpy(h->d_csc_ptr, hsc.csc_ptr.data(), sizeof(int32_t) * ((size_t)N + 1), H2D), "copy csc") &&
          (E == 0 || hip_ok(hipMemcpy(h->d_delay_csc, hsc.delay_csc.data(), E, H2D), "copy csc delays"));
-    if (ok && cfg.layout == LEV_LAYOUT_SLOT_MAJOR)
+    if (ok && cfg.layout == LEV_LAYOUT_SLOT_MAJOR && !scatter_window)
       ok = dalloc(h, &h->d_dsrc, E) &&
            (E == 0 || hip_ok(hipMemcpy(h->d_dsrc, hsc.dsrc.data(), sizeof(uint16_t) * E, H2D), "copy edge sources"));
     const size_t ne = early_src.size();
@@ -892,7 +899,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     if (!ok) return fail();
     if (want_binned) {
       if (!dalloc(h, &h->d_pos, E) || !dalloc_edges(h, &h->d_hpos, E) ||
-          !upload_runs(h, bin_rank, bin_runs, &h->d_rank_run, &h->d_run_ptr, &h->d_run_base) ||
+          !upload_runs(h, bin_rank, bin_runs, &h->d_rank_run, &h->d_run_ptr, &h->d_run_base,
+                       scatter_window ? &hsc : nullptr) ||
           !dalloc(h, &h->d_early_pos, ne))
         return fail();
       if (ne && !hip_ok(hipMemcpy(h->d_early_pos, bin_early_pos.data(), sizeof(uint32_t) * ne, H2D), "copy early"))
@@ -920,7 +928,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         const lev::HostRegroup& rg = st_rg[q];
         ps.n_rg_chunks = (int32_t)rg.chunks.size();
         if (!dalloc(h, &ps.d_pos, E) || !dalloc_edges(h, &ps.d_lpos, E) ||
-            !upload_runs(h, st_rank[q], st_runs[q], &ps.d_rank_run, &ps.d_run_ptr, &ps.d_run_base) ||
+            !upload_runs(h, st_rank[q], st_runs[q], &ps.d_rank_run, &ps.d_run_ptr, &ps.d_run_base,
+                         scatter_window ? &hsc : nullptr) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
             !dalloc(h, &ps.d_rg_runs, rg.runs.size()) || !dalloc(h, &ps.d_early_pos, ne))
           return fail();

@@ -79,12 +79,12 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   static_assert(MODE != kScatterWindow || LAYOUT == 1, "the window form reads a slot-major ring");
   static_assert(MODE != kScatterOff32 || LAYOUT == 0, "32-bit offsets are used for node-major rings");
   constexpr int WSLOTS = scatter_window_slots(sizeof(H));
-  constexpr size_t kValBytes = sizeof(V) * (CAP + 1);
-  constexpr size_t kWinBytes = MODE == kScatterWindow ? (size_t)WSLOTS * kScatterWinCols * sizeof(H) : 0;
+  // the block's value vectors in slot order (gathers); the window form's window of the ring is
+  // dynamic shared memory of S * kScatterWinCols elements
+  constexpr size_t kRawBytes = MODE == kScatterWindow ? 16 : sizeof(V) * (CAP + 1);
   __shared__ uint8_t s_esrc[LAYOUT == 0 ? CAP : 1];  // node-major: each edge's source in the block
-  // the block's value vectors in slot order; before them, in the window form,
-  // the ring's window of the block's sources
-  __shared__ __attribute__((aligned(16))) char s_raw[kValBytes > kWinBytes ? kValBytes : kWinBytes];
+  __shared__ __attribute__((aligned(16))) char s_raw[kRawBytes];
+  extern __shared__ __attribute__((aligned(16))) char s_dyn[];
   V* const s_val = reinterpret_cast<V*>(s_raw);
   __shared__ int32_t s_base[kScatterRuns];  // slot bases of the block's first runs
   const int tid = threadIdx.x;
@@ -99,193 +99,233 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
   const int64_t pitch = a.ring_pitch;
 
   if (!(blk.w & kBlockLongRow)) {
-    if constexpr (LAYOUT == 0) {
-      // node-major: the block's out-edges are in source order; map each edge to its source
-      const int32_t nsrc = (blk.w >> 1) & 511;
-      int32_t rs = 0, re = 0;
-      if (tid < nsrc) {
-        rs = a.csc_ptr[s0 + tid] - c0;
-        re = a.csc_ptr[s0 + tid + 1] - c0;
-      }
-      // a source with more than kSerialRow out-edges is mapped by its whole wavefront
-      const bool heavy = tid < nsrc && re - rs > kSerialRow;
-      if (tid < nsrc && !heavy)
-        for (int32_t q = rs; q < re; ++q) s_esrc[q] = (uint8_t)tid;
-      const int lane = tid & 63;
-      for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
-        const int src = __ffsll((unsigned long long)m) - 1;
-        const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
-        for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
-      }
-      __syncthreads();
-    }
-    // Every load is unconditional (a lane past the block's edges reads out-edge
-    // 0 and the block's first source, and discards the values), so all of a
-    // lane's loads are in flight at once; with a branch around each one the
-    // compiler waited for every history load before issuing the next.
-    // the block's runs of consecutive slots: slot-order index k of run r goes
-    // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
-    const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
-    int32_t tb[kScatterRuns / T];
-#pragma unroll
-    for (int j = 0; j < kScatterRuns / T; ++j) {
-      const int32_t r = j * T + tid;
-      tb[j] = a.run_base[r < nr ? rb + r : 0];
-    }
-    // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
-    int32_t dd[EPT];
-    uint32_t rk[EPT];  // rank | run index << 16
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * T + tid;
-      // unsigned 32-bit element indices: the loads take the scalar-base form
-      const uint32_t c = (uint32_t)(le < nnz ? c0 + le : 0);
-      if constexpr (LAYOUT == 0) {
-        dd[k] = __builtin_nontemporal_load(a.delay + c);
-      } else {
-        const int32_t code = __builtin_nontemporal_load(a.dsrc + c);
-        dd[k] = le < nnz ? code : 0;
-      }
-      rk[k] = __builtin_nontemporal_load(a.rank_run + c);
-    }
-    // node-major: edges in source order, so the lanes of one source share its
-    // history line (read with the streaming hint: only this block reuses it,
-    // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges
-    // in (delay, source) order, so the lanes of one load read one slot row for
-    // consecutive sources; for q > 0 those rows were read for q - 1 by the
-    // neighbouring delay and are in cache.
-    // the K steps' values of every edge, all loads in flight at once, packed per edge
-    V pv[EPT];
-    if constexpr (MODE == kScatterOff32) {
-      // slot of step t + q is slot0 + q, less S once it passes the ring's end
-      const char* base = reinterpret_cast<const char*>(a.ring);
-      const uint32_t wrap_bytes = (uint32_t)S * (uint32_t)sizeof(H);  // a whole turn of the ring
-      uint32_t off0[EPT];
-      int32_t slot0[EPT];
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * T + tid;
-        const int32_t d = dd[k] & 255;
-        slot0[k] = cur - d + (cur < d ? S : 0);  // slot of step t: -S < cur - d < S
-        const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-        off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
-      }
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        H v[EPT];
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-          const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? wrap_bytes : 0u);
-          v[k] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
-        }
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
-      }
-    } else if constexpr (MODE == kScatterWindow) {
-      // The block's sources [s0, s0 + nsrc) in every slot row: an 8-aligned
-      // window of kScatterWinCols columns per slot comes in with coalesced
-      // 16-byte loads, which wait for nothing but the block's descriptor (the
-      // gathers wait for the edge codes first), and each edge reads its K
-      // values from LDS.  The ring holds kScatterWinCols elements of padding
-      // past its end for the last slot row's window.
-      H* const s_win = reinterpret_cast<H*>(s_raw);
+    if constexpr (MODE == kScatterWindow) {
+      // Window form (slot-major ring).  The block's out-edges are stored in the
+      // order of their value slots, so consecutive lanes write consecutive slots
+      // of a run straight from registers: no rank, no placement in LDS.  Each
+      // edge's code holds its source in the block << 8 | delay (low half) and
+      // its run of consecutive slots (high half).  The block's sources
+      // [s0, s0 + nsrc) in every slot row -- an 8-aligned window of
+      // kScatterWinCols columns per slot -- come in with coalesced 16-byte
+      // loads that wait for nothing but the block's descriptor, and each edge
+      // reads its K values from LDS.  The ring holds kScatterWinCols elements of
+      // padding past its end for the last slot row's window.
+      H* const s_win = reinterpret_cast<H*>(s_dyn);
       constexpr int EPC = 16 / (int)sizeof(H);          // elements per 16-byte chunk
       constexpr int CPR = kScatterWinCols / EPC;        // chunks per window row
       constexpr int NCH = (WSLOTS * CPR + T - 1) / T;   // chunk loads per lane at most
+      const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
+      int32_t tb[kScatterRuns / T];
+  #pragma unroll
+      for (int j = 0; j < kScatterRuns / T; ++j) {
+        const int32_t r = j * T + tid;
+        tb[j] = a.run_base[r < nr ? rb + r : 0];
+      }
+      uint32_t code[EPT];
+  #pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;
+        code[k] = __builtin_nontemporal_load(a.rank_run + (uint32_t)(le < nnz ? c0 + le : 0));
+      }
       const int32_t w0 = s0 & ~7;
       const int32_t nch = S * CPR;
       uint4 ch[NCH];
-#pragma unroll
+  #pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int32_t id = j * T + tid < nch ? j * T + tid : 0;
         const int32_t r = id / CPR, c = id % CPR;
         ch[j] = *reinterpret_cast<const uint4*>(ring + (int64_t)r * pitch + w0 + c * EPC);
       }
-#pragma unroll
+  #pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int32_t id = j * T + tid;
         if (id < nch) *reinterpret_cast<uint4*>(s_win + (id / CPR) * kScatterWinCols + (id % CPR) * EPC) = ch[j];
       }
+  #pragma unroll
+      for (int j = 0; j < kScatterRuns / T; ++j) {
+        const int32_t r = j * T + tid;
+        if (r < nr) s_base[r] = tb[j];
+      }
       __syncthreads();
       const int32_t col0 = s0 - w0;
-#pragma unroll
+      V pv[EPT];
+  #pragma unroll
       for (int q = 0; q < K; ++q) {
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-          const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
-          pv[k].x[q] = s_win[slot * kScatterWinCols + col0 + (dd[k] >> 8)];
+          const int32_t slot = ahead_slot(cur, q, (int32_t)(code[k] & 255u), S);
+          pv[k].x[q] = s_win[slot * kScatterWinCols + col0 + (int32_t)((code[k] >> 8) & 255u)];
         }
       }
-      __syncthreads();  // every window value is read before the placement below overwrites them
-    } else {
-      // One step's loads go into registers of their own and are packed
-      // afterwards.  (Small changes of this code's shape moved the pass a lot:
-      // loading straight into the vectors made the compiler issue the uint16
-      // history loads one at a time, 2.2 instead of 1.6 ms per four-step
-      // pass, and a lambda that returned the source node took the FP32
-      // two-step pass from 1.33 to 1.51 ms, both at BA N = 10^7.)
-      auto load_values = [&](H (&v)[EPT], int q) {
-#pragma unroll
+      // the block's runs of consecutive slots: index le of run r goes to slot run_base[r] + le
+      // (a lane past the block's edges loaded edge 0's code and stores nothing)
+      if (nr_all <= kScatterRuns) {
+        int32_t at[EPT];
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k) at[k] = s_base[(code[k] >> 16) & (kScatterRuns - 1)] + k * T + tid;
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k)
+          if (k * T + tid < nnz) hbuf[at[k]] = pv[k];
+      } else {
+  #pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * T + tid;
-          const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
-          int32_t node;
-          if constexpr (LAYOUT == 0) {
-            node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
-          } else {
-            node = s0 + (dd[k] >> 8);
-          }
-          const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
-          v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+          if (le < nnz) hbuf[a.run_base[rb + (int32_t)(code[k] >> 16)] + le] = pv[k];
         }
-      };
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        H v[EPT];
-        load_values(v, q);
-#pragma unroll
-        for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
       }
-    }
-    // place in slot order; a lane past the block's edges writes the spare
-    // slot CAP, so that no load is sunk into a branch and waited for alone
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int32_t le = k * T + tid;
-      s_val[le < nnz ? (rk[k] & 0xffff) : CAP] = pv[k];
-    }
-#pragma unroll
-    for (int j = 0; j < kScatterRuns / T; ++j) {
-      const int32_t r = j * T + tid;
-      if (r < nr) s_base[r] = tb[j];
-    }
-    __syncthreads();
-    // write in slot order: the values bound for one bucket leave as one contiguous run.
-    // The branch on the run count is uniform over the block: with a per-lane
-    // choice between s_base and run_base the compiler read both through one
-    // flat load of a selected pointer, and every store then waited for all
-    // earlier loads and stores (vmcnt(0)).
-    // All slots and values are read from LDS first, into registers of their
-    // own, and the stores follow: with one register set reused per store,
-    // each store's source registers were guarded by a wait for every
-    // earlier store (vmcnt(0)).
-    if (nr_all <= kScatterRuns) {
-      int32_t at[EPT];
-      V val[EPT];
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
-        at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
-        val[k] = s_val[le];
-      }
-#pragma unroll
-      for (int k = 0; k < EPT; ++k)
-        if (k * T + tid < nnz) hbuf[at[k]] = val[k];
     } else {
-#pragma unroll
+      if constexpr (LAYOUT == 0) {
+        // node-major: the block's out-edges are in source order; map each edge to its source
+        const int32_t nsrc = (blk.w >> 1) & 511;
+        int32_t rs = 0, re = 0;
+        if (tid < nsrc) {
+          rs = a.csc_ptr[s0 + tid] - c0;
+          re = a.csc_ptr[s0 + tid + 1] - c0;
+        }
+        // a source with more than kSerialRow out-edges is mapped by its whole wavefront
+        const bool heavy = tid < nsrc && re - rs > kSerialRow;
+        if (tid < nsrc && !heavy)
+          for (int32_t q = rs; q < re; ++q) s_esrc[q] = (uint8_t)tid;
+        const int lane = tid & 63;
+        for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+          const int src = __ffsll((unsigned long long)m) - 1;
+          const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+          for (int32_t q = hs + lane; q < he; q += 64) s_esrc[q] = (uint8_t)((tid & ~63) + src);
+        }
+        __syncthreads();
+      }
+      // Every load is unconditional (a lane past the block's edges reads out-edge
+      // 0 and the block's first source, and discards the values), so all of a
+      // lane's loads are in flight at once; with a branch around each one the
+      // compiler waited for every history load before issuing the next.
+      // the block's runs of consecutive slots: slot-order index k of run r goes
+      // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
+      const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
+      int32_t tb[kScatterRuns / T];
+  #pragma unroll
+      for (int j = 0; j < kScatterRuns / T; ++j) {
+        const int32_t r = j * T + tid;
+        tb[j] = a.run_base[r < nr ? rb + r : 0];
+      }
+      // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
+      int32_t dd[EPT];
+      uint32_t rk[EPT];  // rank | run index << 16
+  #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
-        if (le < nnz) hbuf[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = s_val[le];
+        // unsigned 32-bit element indices: the loads take the scalar-base form
+        const uint32_t c = (uint32_t)(le < nnz ? c0 + le : 0);
+        if constexpr (LAYOUT == 0) {
+          dd[k] = __builtin_nontemporal_load(a.delay + c);
+        } else {
+          const int32_t code = __builtin_nontemporal_load(a.dsrc + c);
+          dd[k] = le < nnz ? code : 0;
+        }
+        rk[k] = __builtin_nontemporal_load(a.rank_run + c);
+      }
+      // node-major: edges in source order, so the lanes of one source share its
+      // history line (read with the streaming hint: only this block reuses it,
+      // and it leaves the L2 to the buckets' write fronts).  Slot-major: edges
+      // in (delay, source) order, so the lanes of one load read one slot row for
+      // consecutive sources; for q > 0 those rows were read for q - 1 by the
+      // neighbouring delay and are in cache.
+      // the K steps' values of every edge, all loads in flight at once, packed per edge
+      V pv[EPT];
+      if constexpr (MODE == kScatterOff32) {
+        // slot of step t + q is slot0 + q, less S once it passes the ring's end
+        const char* base = reinterpret_cast<const char*>(a.ring);
+        const uint32_t wrap_bytes = (uint32_t)S * (uint32_t)sizeof(H);  // a whole turn of the ring
+        uint32_t off0[EPT];
+        int32_t slot0[EPT];
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;
+          const int32_t d = dd[k] & 255;
+          slot0[k] = cur - d + (cur < d ? S : 0);  // slot of step t: -S < cur - d < S
+          const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+          off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
+        }
+  #pragma unroll
+        for (int q = 0; q < K; ++q) {
+          H v[EPT];
+  #pragma unroll
+          for (int k = 0; k < EPT; ++k) {
+            const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? wrap_bytes : 0u);
+            v[k] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
+          }
+  #pragma unroll
+          for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
+        }
+      } else {
+        // One step's loads go into registers of their own and are packed
+        // afterwards.  (Small changes of this code's shape moved the pass a lot:
+        // loading straight into the vectors made the compiler issue the uint16
+        // history loads one at a time, 2.2 instead of 1.6 ms per four-step
+        // pass, and a lambda that returned the source node took the FP32
+        // two-step pass from 1.33 to 1.51 ms, both at BA N = 10^7.)
+        auto load_values = [&](H (&v)[EPT], int q) {
+  #pragma unroll
+          for (int k = 0; k < EPT; ++k) {
+            const int32_t le = k * T + tid;
+            const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
+            int32_t node;
+            if constexpr (LAYOUT == 0) {
+              node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
+            } else {
+              node = s0 + (dd[k] >> 8);
+            }
+            const H* src = ring + ring_index<LAYOUT>(pitch, node, slot);
+            v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
+          }
+        };
+  #pragma unroll
+        for (int q = 0; q < K; ++q) {
+          H v[EPT];
+          load_values(v, q);
+  #pragma unroll
+          for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
+        }
+      }
+      // place in slot order; a lane past the block's edges writes the spare
+      // slot CAP, so that no load is sunk into a branch and waited for alone
+  #pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * T + tid;
+        s_val[le < nnz ? (rk[k] & 0xffff) : CAP] = pv[k];
+      }
+  #pragma unroll
+      for (int j = 0; j < kScatterRuns / T; ++j) {
+        const int32_t r = j * T + tid;
+        if (r < nr) s_base[r] = tb[j];
+      }
+      __syncthreads();
+      // write in slot order: the values bound for one bucket leave as one contiguous run.
+      // The branch on the run count is uniform over the block: with a per-lane
+      // choice between s_base and run_base the compiler read both through one
+      // flat load of a selected pointer, and every store then waited for all
+      // earlier loads and stores (vmcnt(0)).
+      // All slots and values are read from LDS first, into registers of their
+      // own, and the stores follow: with one register set reused per store,
+      // each store's source registers were guarded by a wait for every
+      // earlier store (vmcnt(0)).
+      if (nr_all <= kScatterRuns) {
+        int32_t at[EPT];
+        V val[EPT];
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
+          at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
+          val[k] = s_val[le];
+        }
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k)
+          if (k * T + tid < nnz) hbuf[at[k]] = val[k];
+      } else {
+  #pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * T + tid;
+          if (le < nnz) hbuf[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = s_val[le];
+        }
       }
     }
   } else {
@@ -589,8 +629,9 @@ static void scatter_launch(int layout, const ScatterArgs& a, hipStream_t s) {
   const bool off32 = a.ring_bytes < (int64_t(1) << 32);
   // (the 32-bit offset gathers were slower than the 64-bit ones on a slot-major ring: BA N = 10^7, FP32,
   // two steps per pass, 0.77 against 0.66 ms per step)
-  if (layout != 0 && a.n_slots <= scatter_window_slots(sizeof(H)))
-    scatter_kernel<H, 1, K, kScatterWindow><<<grid, block, 0, s>>>(a);
+  if (scatter_uses_window(layout, a.n_slots, sizeof(H)))
+    scatter_kernel<H, 1, K, kScatterWindow>
+        <<<grid, block, (size_t)a.n_slots * kScatterWinCols * sizeof(H), s>>>(a);
   else if (layout != 0)
     scatter_kernel<H, 1, K, kScatterGather64><<<grid, block, 0, s>>>(a);
   else if (off32)

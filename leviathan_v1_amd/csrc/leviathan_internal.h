@@ -118,12 +118,19 @@ struct RegroupArgs {
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
 
 // The scatter's window form (slot-major rings): a short source block's <= kBlock
-// sources start at a multiple of 8, so kScatterWinCols columns of every slot row
-// hold them; the window of all S slots must fit kScatterWinBytes of LDS.
-constexpr int kScatterWinCols = kBlock + 16;
+// sources start at most 7 columns past a multiple of 8, so kScatterWinCols
+// columns of every slot row hold them; the window of all S slots must fit
+// kScatterWinBytes of LDS.
+constexpr int kScatterWinCols = kBlock + 8;
 constexpr int kScatterWinBytes = 32768;
 __host__ __device__ constexpr int scatter_window_slots(size_t elem_bytes) {
   return (int)(kScatterWinBytes / (kScatterWinCols * elem_bytes));
+}
+// The window form is used for slot-major rings (layout 1) that fit; its short source
+// blocks store their out-edges in value-slot order with one code per edge
+// (source in block << 8 | delay, run index << 16) in place of the rank and run.
+__host__ __device__ constexpr bool scatter_uses_window(int layout, int n_slots, size_t elem_bytes) {
+  return layout == 1 && n_slots <= scatter_window_slots(elem_bytes);
 }
 constexpr int kScatterRuns = 1024;  // runs of a short source block whose slot bases are staged in LDS
 static_assert(kScatterRuns % kBlock == 0, "run bases are staged kBlock at a time");

@@ -259,6 +259,20 @@ void encode_scatter_runs(const HostScatter& sc, const std::vector<uint32_t>& pos
   });
 }
 
+void window_scatter_codes(const HostScatter& sc, const std::vector<uint16_t>& rank, const HostScatterRuns& runs,
+                          std::vector<uint32_t>& code) {
+  code.assign(rank.size(), 0);
+  parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {
+    const int4 it = sc.sources[q];
+    if (it.w & kBlockLongRow) return;
+    const int32_t c0 = it.y, c1 = it.z;
+    for (int32_t c = c0; c < c1; ++c) {
+      const int32_t k = rank[c];  // the edge's index in its block's slot order
+      code[c0 + k] = (uint32_t)sc.dsrc[c] | (uint32_t)runs.run_idx[c0 + k] << 16;
+    }
+  });
+}
+
 void order_scatter_by_delay(HostScatter& sc) {
   sc.dsrc.assign(sc.delay_csc.size(), 0);
   parallel_for((int64_t)sc.sources.size(), [&](int64_t q) {

@@ -110,6 +110,14 @@ void encode_scatter_runs(const HostScatter& sc, const std::vector<uint32_t>& pos
 // Must run before the positions are derived from csc_edge.
 void order_scatter_by_delay(HostScatter& sc);
 
+// Window form of the scatter (slot-major ring, after order_scatter_by_delay,
+// sort_scatter_writes and encode_scatter_runs): inside each short source
+// block the out-edges are stored in the order of their value slots, one code
+// per edge: its source in the block << 8 | delay (low half) and its run of
+// consecutive slots (high half).  Long sources keep CSC order (code unused).
+void window_scatter_codes(const HostScatter& sc, const std::vector<uint16_t>& rank, const HostScatterRuns& runs,
+                          std::vector<uint32_t>& code);
+
 // Multi-step scatter: the CSR edges with delay < below, sorted by (delay,
 // source, CSR index).  src/delay/csr hold each record's source, delay and CSR
 // edge; cnt[d] (d = 0..below) = the number of records with delay < d.
