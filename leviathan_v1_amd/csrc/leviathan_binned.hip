@@ -116,32 +116,35 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       constexpr int NCH = (WSLOTS * CPR + T - 1) / T;   // chunk loads per lane at most
       const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
       int32_t tb[kScatterRuns / T];
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < kScatterRuns / T; ++j) {
         const int32_t r = j * T + tid;
         tb[j] = a.run_base[r < nr ? rb + r : 0];
       }
       uint32_t code[EPT];
-  #pragma unroll
+#pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
         code[k] = __builtin_nontemporal_load(a.rank_run + (uint32_t)(le < nnz ? c0 + le : 0));
       }
+      // (Loading only the chunks that hold the block's sources, and 16-byte
+      // loads typed as a clang vector rather than uint4, each made the pass
+      // slower: 0.338 -> 0.361 and -> 0.406 ms per step with the early gathers.)
       const int32_t w0 = s0 & ~7;
       const int32_t nch = S * CPR;
       uint4 ch[NCH];
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int32_t id = j * T + tid < nch ? j * T + tid : 0;
         const int32_t r = id / CPR, c = id % CPR;
         ch[j] = *reinterpret_cast<const uint4*>(ring + (int64_t)r * pitch + w0 + c * EPC);
       }
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int32_t id = j * T + tid;
         if (id < nch) *reinterpret_cast<uint4*>(s_win + (id / CPR) * kScatterWinCols + (id % CPR) * EPC) = ch[j];
       }
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < kScatterRuns / T; ++j) {
         const int32_t r = j * T + tid;
         if (r < nr) s_base[r] = tb[j];
@@ -149,9 +152,9 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       __syncthreads();
       const int32_t col0 = s0 - w0;
       V pv[EPT];
-  #pragma unroll
+#pragma unroll
       for (int q = 0; q < K; ++q) {
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t slot = ahead_slot(cur, q, (int32_t)(code[k] & 255u), S);
           pv[k].x[q] = s_win[slot * kScatterWinCols + col0 + (int32_t)((code[k] >> 8) & 255u)];
@@ -161,13 +164,13 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       // (a lane past the block's edges loaded edge 0's code and stores nothing)
       if (nr_all <= kScatterRuns) {
         int32_t at[EPT];
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) at[k] = s_base[(code[k] >> 16) & (kScatterRuns - 1)] + k * T + tid;
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k)
           if (k * T + tid < nnz) hbuf[at[k]] = pv[k];
       } else {
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * T + tid;
           if (le < nnz) hbuf[a.run_base[rb + (int32_t)(code[k] >> 16)] + le] = pv[k];
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       // to slot run_base[r] + k; the first kScatterRuns bases are staged in LDS
       const int32_t rb = a.run_ptr[b], nr_all = a.run_ptr[b + 1] - rb, nr = min(nr_all, kScatterRuns);
       int32_t tb[kScatterRuns / T];
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < kScatterRuns / T; ++j) {
         const int32_t r = j * T + tid;
         tb[j] = a.run_base[r < nr ? rb + r : 0];
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       // node-major: dd holds the delay; slot-major: the source index in the block << 8 | delay
       int32_t dd[EPT];
       uint32_t rk[EPT];  // rank | run index << 16
-  #pragma unroll
+#pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
         // unsigned 32-bit element indices: the loads take the scalar-base form
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         const uint32_t wrap_bytes = (uint32_t)S * (uint32_t)sizeof(H);  // a whole turn of the ring
         uint32_t off0[EPT];
         int32_t slot0[EPT];
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * T + tid;
           const int32_t d = dd[k] & 255;
@@ -245,15 +248,15 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
           const int32_t node = s0 + (le < nnz ? (int32_t)s_esrc[le] : 0);
           off0[k] = ((uint32_t)node * (uint32_t)pitch + (uint32_t)slot0[k]) * (uint32_t)sizeof(H);
         }
-  #pragma unroll
+#pragma unroll
         for (int q = 0; q < K; ++q) {
           H v[EPT];
-  #pragma unroll
+#pragma unroll
           for (int k = 0; k < EPT; ++k) {
             const uint32_t off = off0[k] + (uint32_t)(q * sizeof(H)) - (slot0[k] >= S - q ? wrap_bytes : 0u);
             v[k] = __builtin_nontemporal_load(reinterpret_cast<const H*>(base + off));
           }
-  #pragma unroll
+#pragma unroll
           for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
         }
       } else {
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         // pass, and a lambda that returned the source node took the FP32
         // two-step pass from 1.33 to 1.51 ms, both at BA N = 10^7.)
         auto load_values = [&](H (&v)[EPT], int q) {
-  #pragma unroll
+#pragma unroll
           for (int k = 0; k < EPT; ++k) {
             const int32_t le = k * T + tid;
             const int32_t slot = ahead_slot(cur, q, dd[k] & 255, S);
@@ -278,22 +281,22 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
             v[k] = LAYOUT == 0 ? __builtin_nontemporal_load(src) : *src;
           }
         };
-  #pragma unroll
+#pragma unroll
         for (int q = 0; q < K; ++q) {
           H v[EPT];
           load_values(v, q);
-  #pragma unroll
+#pragma unroll
           for (int k = 0; k < EPT; ++k) pv[k].x[q] = v[k];
         }
       }
       // place in slot order; a lane past the block's edges writes the spare
       // slot CAP, so that no load is sunk into a branch and waited for alone
-  #pragma unroll
+#pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int32_t le = k * T + tid;
         s_val[le < nnz ? (rk[k] & 0xffff) : CAP] = pv[k];
       }
-  #pragma unroll
+#pragma unroll
       for (int j = 0; j < kScatterRuns / T; ++j) {
         const int32_t r = j * T + tid;
         if (r < nr) s_base[r] = tb[j];
@@ -311,17 +314,17 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
       if (nr_all <= kScatterRuns) {
         int32_t at[EPT];
         V val[EPT];
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * T + tid;  // < CAP: every LDS read is in range
           at[k] = s_base[(rk[k] >> 16) & (kScatterRuns - 1)] + le;
           val[k] = s_val[le];
         }
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k)
           if (k * T + tid < nnz) hbuf[at[k]] = val[k];
       } else {
-  #pragma unroll
+#pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * T + tid;
           if (le < nnz) hbuf[a.run_base[rb + (int32_t)(rk[k] >> 16)] + le] = s_val[le];
