@@ -159,7 +159,7 @@ struct LeviathanHandle {
   uint16_t* d_dsrc = nullptr;      // slot-major ring: each CSC edge's source index in its block << 8 | delay
   uint32_t* d_pos = nullptr;
   void* d_hbuf = nullptr;    // two-phase schedules: the scatter kernel's bucketed values
-  void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block, stage_stride() elements per step
+  void* d_hbuf2 = nullptr;   // staged schedule: the values regrouped by row block
   uint32_t* d_hpos = nullptr;
   uint32_t* d_rank_run = nullptr;  // binned plans' ranks and scatter runs
   int32_t* d_run_ptr = nullptr;
@@ -224,24 +224,6 @@ bool dalloc(LeviathanHandle* h, T** p, size_t count) {
   h->vram += bytes;
   return true;
 }
-
-// A per-edge array of the row pass: E elements and kEdgePad zeroed ones past
-// them (a stream block's lanes read whole aligned groups of positions, the
-// last of which may reach past the last edge).
-template <class T>
-bool dalloc_edges(LeviathanHandle* h, T** p, size_t E) {
-  if (!dalloc(h, p, E + lev::kEdgePad)) return false;
-  const hipError_t e = hipMemset(*p + E, 0, sizeof(T) * lev::kEdgePad);
-  if (e != hipSuccess) {
-    set_err("hipMemset failed: %s", hipGetErrorString(e));
-    return false;
-  }
-  return true;
-}
-
-// Elements between consecutive steps' staged arrays: E plus the padding,
-// rounded up to 8 so that every step's array starts 16-byte aligned.
-size_t stage_stride(const LeviathanHandle* h) { return ((size_t)h->E + lev::kEdgePad + 7) & ~(size_t)7; }
 
 // The scatter's per-edge rank and per-slot-order-index run, packed rank | run << 16 (both are
 // indexed by the block's first out-edge + the lane's index), and its run tables.
@@ -427,7 +409,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
   const size_t hsize = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
   // the values step k's row pass reads: the regrouped array of step k (staged), or component k of
   // the bucketed slots, which hold the K steps' values side by side (binned)
-  void* hk = staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)k * stage_stride(h) * hsize)
+  void* hk = staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)k * h->E * hsize)
                     : static_cast<void*>(static_cast<char*>(h->d_hbuf) + (size_t)k * hsize);
   const int32_t hstride = staged ? 1 : K;
   hipError_t e = hipSuccess;
@@ -458,7 +440,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     ra.src = h->d_hbuf;
     ra.dst = h->d_hbuf2;
     ra.num_chunks = pl.n_rg_chunks;
-    ra.stride = (int64_t)stage_stride(h);
+    ra.stride = h->E;
     ra.steps = K;
     e = lev::launch_regroup(h->cfg.history, ra, h->stream);
   }
@@ -838,11 +820,12 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     c.n_nodes = offs[k + 1] - offs[k];
   }
 
-  if (!dalloc(h, &h->d_row_ptr, (size_t)N + 1) || !dalloc_edges(h, &h->d_col, (size_t)h->E) ||
-      !dalloc_edges(h, &h->d_delay, (size_t)h->E) || !dalloc_edges(h, &h->d_w, (size_t)h->E) ||
+  if (!dalloc(h, &h->d_row_ptr, (size_t)N + 1) || !dalloc(h, &h->d_col, (size_t)h->E) ||
+      !dalloc(h, &h->d_delay, (size_t)h->E) || !dalloc(h, &h->d_w, (size_t)h->E) ||
       !dalloc(h, &h->d_theta, (size_t)N) || !dalloc(h, &h->d_theta_hat, (size_t)N) ||
       !dalloc(h, &h->d_omega, (size_t)N) || !dalloc(h, &h->d_omega_hat, (size_t)N) ||
-      !dalloc(h, (char**)&h->d_ring, (ring_elems + lev::kScatterWinCols) * hsize) || !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
+      !dalloc(h, (char**)&h->d_ring, (ring_elems + lev::kScatterWinCols) * hsize) ||
+      !dalloc(h, &h->d_partials, (size_t)h->n_partials) ||
       !dalloc(h, &h->d_chunks, (size_t)(h->n_partials + lev::kChunkPartials - 1) / lev::kChunkPartials) ||
       !dalloc(h, &h->d_fin_done, 0) ||
       !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
@@ -883,7 +866,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
     const size_t E = (size_t)h->E;
     if (!dalloc(h, &h->d_sources, hsc.sources.size()) || !dalloc(h, &h->d_csc_ptr, (size_t)N + 1) ||
         !dalloc(h, &h->d_delay_csc, E) || !dalloc(h, (char**)&h->d_hbuf, E * hist_bytes) ||
-        (want_staged && !dalloc(h, (char**)&h->d_hbuf2, stage_stride(h) * hist_bytes)))
+        (want_staged && !dalloc(h, (char**)&h->d_hbuf2, E * hist_bytes)))
       return fail();
     ok = hip_ok(hipMemcpy(h->d_sources, hsc.sources.data(), sizeof(int4) * hsc.sources.size(), H2D),
                 "copy sources") &&
@@ -898,7 +881,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
                       hip_ok(hipMemcpy(h->d_early_delay, early_delay.data(), ne, H2D), "copy early")));
     if (!ok) return fail();
     if (want_binned) {
-      if (!dalloc(h, &h->d_pos, E) || !dalloc_edges(h, &h->d_hpos, E) ||
+      if (!dalloc(h, &h->d_pos, E) || !dalloc(h, &h->d_hpos, E) ||
           !upload_runs(h, bin_rank, bin_runs, &h->d_rank_run, &h->d_run_ptr, &h->d_run_base,
                        scatter_window ? &hsc : nullptr) ||
           !dalloc(h, &h->d_early_pos, ne))
@@ -927,7 +910,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
         ps.owns_pos = true;
         const lev::HostRegroup& rg = st_rg[q];
         ps.n_rg_chunks = (int32_t)rg.chunks.size();
-        if (!dalloc(h, &ps.d_pos, E) || !dalloc_edges(h, &ps.d_lpos, E) ||
+        if (!dalloc(h, &ps.d_pos, E) || !dalloc(h, &ps.d_lpos, E) ||
             !upload_runs(h, st_rank[q], st_runs[q], &ps.d_rank_run, &ps.d_run_ptr, &ps.d_run_base,
                          scatter_window ? &hsc : nullptr) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
@@ -1560,15 +1543,13 @@ bool time_trial(LeviathanHandle* h, StateSnapshot& snap, float dt, int steps, hi
 
 bool resize_hbuf(LeviathanHandle* h, int steps) {
   if (steps <= h->hbuf_steps) return true;
-  const size_t hsize = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  // per step: the bucketed values, and the staged schedule's regrouped ones
-  const size_t one[2] = {std::max<size_t>((size_t)h->E * hsize, 1), stage_stride(h) * hsize};
+  const size_t one = std::max<size_t>((size_t)h->E * (h->cfg.history == LEV_HISTORY_F32 ? 4 : 2), 1);
   void* bufs[2] = {nullptr, nullptr};
   const int nb = h->d_hbuf2 ? 2 : 1;  // the regrouped values of the staged schedule too
   for (int i = 0; i < nb; ++i) {
-    const hipError_t e = hipMalloc(&bufs[i], one[i] * (size_t)steps);
+    const hipError_t e = hipMalloc(&bufs[i], one * (size_t)steps);
     if (e != hipSuccess) {
-      set_err("hipMalloc(%zu bytes) for the values of %d steps failed: %s", one[i] * (size_t)steps, steps,
+      set_err("hipMalloc(%zu bytes) for the values of %d steps failed: %s", one * (size_t)steps, steps,
               hipGetErrorString(e));
       if (bufs[0]) (void)hipFree(bufs[0]);
       return false;
@@ -1589,7 +1570,7 @@ bool resize_hbuf(LeviathanHandle* h, int steps) {
     (void)hipFree(h->d_hbuf2);
     h->d_hbuf2 = bufs[1];
   }
-  for (int i = 0; i < nb; ++i) h->vram += one[i] * (size_t)(steps - h->hbuf_steps);
+  h->vram += (size_t)nb * one * (size_t)(steps - h->hbuf_steps);
   h->hbuf_steps = steps;
   return true;
 }

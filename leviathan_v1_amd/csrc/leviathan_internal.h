@@ -14,7 +14,6 @@ constexpr int kFinalizeThreads = 1024;
 constexpr int kRHistSteps = 4096;    // steps of r kept on the device ring
 constexpr int kDriveSteps = 4096;    // steps of drive input u_t kept on the device ring
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
-constexpr int kEdgePad = 16;         // per-edge arrays of the row pass: elements allocated past the last edge (>= the widest lane group - 1)
 
 // Row block flag: a single row whose in-degree exceeds the block capacity.
 constexpr int kBlockLongRow = 1;
@@ -24,9 +23,9 @@ __host__ __device__ inline int4 pack_item(int32_t r0, int32_t e0, int32_t e1, in
                                           bool is_long) {
   return make_int4(r0, e0, e1, (net << 10) | (nrows << 1) | (is_long ? kBlockLongRow : 0));
 }
-constexpr int kSerialRow = 128;      // node-major scatter: a source with more out-edges is mapped by a wavefront
+constexpr int kSerialRow = 128;      // stream blocks: longer rows are filled and summed by a wavefront
 // Estimated row-pass cost of a block (ns of workgroup lifetime), for balancing the XCDs.
-constexpr double kCostBlock = 9860.0, kCostEdge = 7.42, kCostRow = 11.9;
+constexpr double kCostBlock = 9860.0, kCostEdge = 7.42, kCostRow = 11.9, kCostHeavyRow = 300.0;
 constexpr double kCostLong = 20900.0, kCostLongEdge = 5.59;
 
 // Per-network controller state, read by the step kernel (g) and updated by
@@ -70,8 +69,6 @@ struct StepArgs {
   const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
   int32_t hstride;        // binned schedule: elements per hbuf slot (the K interleaved steps of a scatter pass)
   const uint16_t* lpos;   // staged schedule: [E] position of each CSR edge's value in its block's range
-  // col, delay, w, hpos, lpos and the staged values hold kEdgePad elements past the last edge: a
-  // stream block's lanes read whole aligned groups of edges
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
 

@@ -64,38 +64,16 @@ __device__ __forceinline__ Item unpack_item(int4 v) {
   return it;
 }
 
-// Wide access to N consecutive elements at an index that is a multiple of
-// min(N, 8): a stream block's window starts at a multiple of 8 edges and lane
-// tid's positions at tid * N, so these loads and stores are aligned.
-template <class T, int N>
-constexpr int pack_align() {
-  return (int)sizeof(T) * (N < 8 ? N : 8) < 16 ? (int)sizeof(T) * (N < 8 ? N : 8) : 16;
-}
-template <class T, int N>
-struct alignas(pack_align<T, N>()) Pack {
-  T v[N];
-};
-template <class T, int N>
-__device__ __forceinline__ Pack<T, N> load_pack(const T* p) {
-  return *reinterpret_cast<const Pack<T, N>*>(p);
-}
-
 template <class H, int LAYOUT, int EPT, int GATHER>
 __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   constexpr int CAP = kBlock * EPT;
-  constexpr int NW = kBlock / 64;     // wavefronts per workgroup
-  constexpr int WSPAN = 64 * EPT;     // window positions per wavefront
-  constexpr int32_t kNoRow = kBlock;  // row id of the positions past the block's edges
-  __shared__ __attribute__((aligned(16))) uint16_t s_mark[CAP];  // row starts, then nothing
-  __shared__ __attribute__((aligned(16))) H s_val[GATHER == 2 ? CAP : 8];
-  __shared__ float2 s_tg[kBlock + 1];     // per row: theta (revolutions for uint16) and gate
-  __shared__ double s_rowsum[kBlock + 1];  // per row: coupling sum of its last wavefront's part
-  __shared__ double s_wcarry[NW];          // [w]: part of the row crossing into wavefront w
-  __shared__ int32_t s_wrow[NW];           // [w]: row holding wavefront w's first position
-  __shared__ double s_red[2][NW];
+  __shared__ float s_theta[kBlock];
+  __shared__ float s_gate[kBlock];
+  __shared__ uint8_t s_erow[CAP];
+  __shared__ float s_contrib[CAP];
+  __shared__ double s_red[2][kBlock / 64];
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
   const int32_t x = blockIdx.x % kNumXcd;
   const int32_t b = a.xcd_split[x] + (int32_t)(blockIdx.x / kNumXcd);
   if (b >= a.xcd_split[x + 1]) return;  // this XCD's run is shorter than the longest
@@ -124,192 +102,139 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 
   if (!it.is_long) {
     // ---------------- stream mode ----------------
-    // The block's edges [e0, e1) are seen through a window of CAP positions
-    // that starts at e0 rounded down to 8 edges; lane tid owns the EPT
-    // consecutive positions [EPT * tid, EPT * tid + EPT) and moves their
-    // weights, indices and staged values with wide loads and stores.  The
-    // positions before off = e0 - base and from npos = off + nnz on are not
-    // the block's ("dead"): they compute on defined values and store nothing.
-    //
-    // Each position learns its row from a prefix maximum of row-start marks
-    // (row ids ascend with the position), and the rows' coupling sums are an
-    // FP64 sum per run of equal row ids inside each lane, joined across the
-    // lanes by a segmented scan: no lane loops over a whole row, so a
-    // wavefront's time no longer follows its longest row.
-    const int32_t base = e0 & ~7;
-    const int32_t off = e0 - base;
-    const int32_t npos = off + nnz;  // <= CAP: the plan's windows hold their blocks
-    const int32_t p0 = tid * EPT;
-    // the loads are unconditional: a lane past the block's edges reads the
-    // window's first positions and discards them
-    const int32_t lb = base + (p0 < npos ? p0 : 0);
-
     RowState rs_;
     int32_t rs = 0, re = 0;
-    float om = 0.0f;  // natural frequency (plus drive), loaded with the row's state
+    float om = 0.0f;  // natural frequency (plus drive), loaded with the row's state: a load after
+                      // the last barrier would add a memory round trip to every row block
     if (tid < nrows) {
       const int32_t i = r0 + tid;
       om = a.omega[i];
       if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
-      rs = a.row_ptr[i] - base;
-      re = a.row_ptr[i + 1] - base;
+      s_theta[tid] = sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th;  // uint16: revolutions (edge_update_rev)
+      s_gate[tid] = rs_.gate;
+      rs = a.row_ptr[i] - e0;
+      re = a.row_ptr[i + 1] - e0;
     }
-    const Pack<float, EPT> wv = load_pack<float, EPT>(a.w + lb);
+    // Rows of up to kSerialRow edges are handled by their own lane, longer
+    // ones by their whole wavefront, one after another (a lane looping over a
+    // long row would hold its wavefront for the row's whole length).
+    const int lane = tid & 63;
+    const bool heavy = tid < nrows && re - rs > kSerialRow;
+    if (tid < nrows && !heavy)
+      for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
+    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+      for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
+    }
+
+    // The loads are unconditional: a lane past the block's edges reads edge 0
+    // (every edge array holds at least one element) and discards the value.
+    // Straight-line code keeps all of a lane's loads in flight at once; with a
+    // branch around each load the compiler waited for every load before
+    // issuing the next.
     int32_t jj[EPT];
     int32_t dd[EPT];
-    if constexpr (GATHER == 0) {
-      const Pack<int32_t, EPT> cv = load_pack<int32_t, EPT>(a.col + lb);
-      const Pack<uint8_t, EPT> dv = load_pack<uint8_t, EPT>(a.delay + lb);
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        jj[k] = cv.v[k];
-        dd[k] = dv.v[k];
-      }
-    } else if constexpr (GATHER == 1) {
-      const Pack<uint32_t, EPT> hv = load_pack<uint32_t, EPT>(a.hpos + lb);
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) jj[k] = (int32_t)hv.v[k];
-    } else {
-      const Pack<uint16_t, EPT> lv = load_pack<uint16_t, EPT>(a.lpos + lb);
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) jj[k] = lv.v[k];
-    }
-    Pack<H, EPT> sv{};
-    if constexpr (GATHER == 2) sv = load_pack<H, EPT>(hbuf + lb);
-    bool live[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) live[k] = p0 + k >= off && p0 + k < npos;
-
-    *reinterpret_cast<Pack<uint16_t, EPT>*>(&s_mark[p0]) = Pack<uint16_t, EPT>{};
-    if (tid < NW) s_wrow[tid] = kNoRow;
-    __syncthreads();
-    if (tid < nrows) {
-      s_tg[tid] = make_float2(sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th, rs_.gate);
-      if (re > rs) s_mark[rs] = (uint16_t)tid;
-#pragma unroll
-      for (int w = 1; w < NW; ++w)
-        if (rs <= w * WSPAN && w * WSPAN < re) s_wrow[w] = tid;
-    }
-    if (tid == 0 && npos < CAP) s_mark[npos] = (uint16_t)kNoRow;
-    if constexpr (GATHER == 2) *reinterpret_cast<Pack<H, EPT>*>(&s_val[p0]) = sv;
-    __syncthreads();
-
-    // row id of each position: the largest mark at or before it
-    const Pack<uint16_t, EPT> mk = *reinterpret_cast<const Pack<uint16_t, EPT>*>(&s_mark[p0]);
-    int32_t mloc = 0;
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) mloc = max(mloc, (int32_t)mk.v[k]);
-    int32_t incl = mloc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl = max(incl, t);
-    }
-    int32_t row = __shfl_up(incl, 1, 64);
-    const int32_t wave_first = wave == 0 ? 0 : s_wrow[wave];
-    row = lane == 0 ? wave_first : max(row, wave_first);
-    int32_t rid[EPT];
+    float ww[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      row = max(row, (int32_t)mk.v[k]);
-      rid[k] = row;
+      const int32_t le = k * kBlock + tid;
+      const int32_t e = le < nnz ? e0 + le : 0;
+      if constexpr (GATHER == 0) {
+        jj[k] = a.col[e];
+        dd[k] = a.delay[e];
+      } else if constexpr (GATHER == 1) {
+        jj[k] = (int32_t)a.hpos[e];
+      } else {
+        jj[k] = (int32_t)a.lpos[e];
+      }
+      ww[k] = a.w[e];
     }
+    if constexpr (GATHER == 2) {
+      // s_contrib holds the block's staged values until the edge pass below
+      H sv[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        sv[k] = hbuf[le < nnz ? e0 + le : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        if (le < nnz) s_contrib[le] = sizeof(H) == 2 ? (float)sv[k] : HistCodec<H>::decode(sv[k]);  // uint16: the code
+      }
+    }
+    __syncthreads();
 
-    // the delayed phases (a dead position reads a valid element and discards it)
     float hh[EPT];
     if constexpr (GATHER == 2) {
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
-        const H v = s_val[live[k] ? off + jj[k] : 0];
-        hh[k] = sizeof(H) == 2 ? (float)v : HistCodec<H>::decode(v);  // uint16: the code
+        const int32_t le = k * kBlock + tid;
+        hh[k] = le < nnz ? s_contrib[jj[k]] : 0.0f;
       }
     } else {
+      // every lane's column, delay or slot is a valid one (edge 0 for a lane past the edges)
       H hr[EPT];
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         if constexpr (GATHER == 0) {
-          int32_t slot = cur - (live[k] ? dd[k] : 0);
+          int32_t slot = cur - dd[k];
           slot += (slot < 0) ? S : 0;
-          hr[k] = ring[ring_index<LAYOUT>(pitch, live[k] ? jj[k] : 0, slot)];
+          hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
         } else {
           // the bucketed array keeps a scatter pass's steps side by side: hstride values per slot
-          hr[k] = hbuf[(int64_t)(uint32_t)(live[k] ? jj[k] : 0) * a.hstride];
+          hr[k] = hbuf[(int64_t)(uint32_t)jj[k] * a.hstride];
         }
       }
 #pragma unroll
       for (int k = 0; k < EPT; ++k) hh[k] = sizeof(H) == 2 ? (float)hr[k] : HistCodec<H>::decode(hr[k]);
     }
-
-    // the edge updates; the coupling uses the pre-update weight (:154)
-    float wn[EPT], cc[EPT];
+    if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
+    // the lane's weight stores address the weights by a 32-bit element index
+    // from the kernel argument's base (E < 2^31), so no store needs 64-bit
+    // address arithmetic of its own
+    const uint32_t wlane = (uint32_t)(e0 + tid);
+    // The edge arithmetic runs on every lane (a lane past the block's edges
+    // works on row 0 and a defined value and stores nothing), so that only the
+    // two stores are predicated and the constants stay in registers across the
+    // unrolled edges.
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const float2 tg = s_tg[rid[k]];
+      const int32_t le = k * kBlock + tid;  // < CAP: the LDS read is in range
+      const bool live = le < nnz;
+      const int lr = live ? (int)s_erow[le] : 0;
       float c;
+      float wn;
       if constexpr (sizeof(H) == 2)
-        wn[k] = edge_update_rev(hh[k], tg.x, tg.y, wv.v[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        wn = edge_update_rev(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
       else
-        wn[k] = edge_update(hh[k], tg.x, tg.y, wv.v[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
-      cc[k] = live[k] ? c : 0.0f;
-    }
-    float* wp = a.w + (uint32_t)(base + p0);
-    if (p0 >= off && p0 + EPT <= npos) {
-      Pack<float, EPT> o;
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) o.v[k] = wn[k];
-      *reinterpret_cast<Pack<float, EPT>*>(wp) = o;
-    } else {
-#pragma unroll
-      for (int k = 0; k < EPT; ++k)
-        if (live[k]) wp[k] = wn[k];
-    }
-
-    // Coupling sums in FP64.  Inside the lane, one sum per run of equal row
-    // ids: a run that ends inside the lane is complete (the first one may
-    // continue one from the lanes before).  The lane's last run is joined
-    // with the lanes before it that end in the same row by a segmented
-    // inclusive scan over the wavefront.  Each row's sum is written by the
-    // lane holding its last position; a row that crosses into the next
-    // wavefront leaves that part in s_wcarry, and its row lane adds them.
-    double run = 0.0, head = 0.0;
-    bool closed = false;  // the lane's first run ended inside the lane
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      run += (double)cc[k];
-      if (k + 1 < EPT && rid[k + 1] != rid[k]) {
-        if (closed) s_rowsum[rid[k]] = run;
-        head = closed ? head : run;
-        closed = true;
-        run = 0.0;
+        wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+      if (live) {
+        s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
+        a.w[wlane + k * kBlock] = wn;
       }
     }
-    const int32_t key = rid[EPT - 1];
-    double sc = run;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const double t = __shfl_up(sc, o, 64);
-      const int32_t kp = __shfl_up(key, o, 64);
-      if (lane >= o && kp == key) sc += t;
-    }
-    const double sc_prev = __shfl_up(sc, 1, 64);
-    const int32_t key_prev = __shfl_up(key, 1, 64);
-    const int32_t first_next = __shfl_down(rid[0], 1, 64);
-    if (closed) s_rowsum[rid[0]] = (lane > 0 && key_prev == rid[0]) ? head + sc_prev : head;
-    const int32_t next_row = lane < 63 ? first_next : (wave + 1 < NW ? s_wrow[wave + 1] : -1);
-    if (next_row != key)
-      s_rowsum[key] = sc;
-    else if (lane == 63)
-      s_wcarry[wave + 1] = sc;
     __syncthreads();
 
+    // Row sums in CSR order: an FP64 sum by the row's lane, rounded once (two
+    // VALU instructions per term where Neumaier's FP32 sum took about eight, in
+    // a loop the wave runs for its longest row), or for a long row lane-strided
+    // Neumaier partials combined by a fixed FP64 tree.
     float coupling = 0.0f;
-    if (tid < nrows && re > rs) {
-      double tot = 0.0;
-#pragma unroll
-      for (int w = 1; w < NW; ++w)
-        if (rs < w * WSPAN && w * WSPAN < re) tot += s_wcarry[w];
-      coupling = (float)(tot + s_rowsum[tid]);
+    if (tid < nrows && !heavy) {
+      double acc = 0.0;
+      for (int32_t q = rs; q < re; ++q) acc += (double)s_contrib[q];
+      coupling = (float)acc;
+    }
+    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+      CompSum acc;
+      for (int32_t q = hs + lane; q < he; q += 64) acc.add(s_contrib[q]);
+      const double tot = wave_sum((double)acc.s + (double)acc.c);
+      if (lane == src) coupling = (float)tot;
     }
 
     if (tid < nrows) {
@@ -333,12 +258,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       om = a.omega[i];
       if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
       rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
-      s_tg[0] = make_float2(rs_.th, rs_.gate);
+      s_theta[0] = rs_.th;
+      s_gate[0] = rs_.gate;
     }
     __syncthreads();
-    const float th_i = s_tg[0].x;
+    const float th_i = s_theta[0];
     const float th_rev = th_i * kInvTwoPi;
-    const float gate = s_tg[0].y;
+    const float gate = s_gate[0];
     CompSum acc;
     for (int32_t base = 0; base < nnz; base += kBlock * 4) {
       // unconditional loads (edge e0 for a lane past the row's end), as in stream mode

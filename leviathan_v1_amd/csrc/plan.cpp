@@ -41,20 +41,16 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
     const int lo = offs[net], hi = offs[net + 1];
     int r = lo;
     while (r < hi) {
-      // a stream block is read through a window of cap edges that starts at
-      // its first edge rounded down to a multiple of 8, so its edges plus that
-      // offset must fit the window
-      const int off = row_ptr[r] & 7;
-      if (off + deg_of(row_ptr, r) > cap) {
+      if (deg_of(row_ptr, r) > cap) {
         longs[net].push_back(pack_item(r, row_ptr[r], row_ptr[r + 1], 1, net, true));
         ++r;
         continue;
       }
       const int start = r;
-      int nnz = off;
+      int nnz = 0;
       while (r < hi && r - start < kBlock) {
         const int d = deg_of(row_ptr, r);
-        if (nnz + d > cap) break;
+        if (d > cap || nnz + d > cap) break;
         nnz += d;
         ++r;
       }
@@ -81,14 +77,21 @@ bool build_direct_plan(const int* row_ptr, int n_nets, const int* offs, int cap,
   }
   // XCD runs of equal estimated cost.  Cost of a block in ns of workgroup
   // lifetime, fitted to per-block timestamps of the row pass on the
-  // Barabasi-Albert 10^7 graph: long rows (strided by the whole workgroup)
-  // cost extra.
+  // Barabasi-Albert 10^7 graph: rows above kSerialRow edges (summed by their
+  // wavefront) and long rows (strided by the whole workgroup) cost extra.
+  // (Fitted with a 32-edge kSerialRow.)
   std::vector<double> cum(out.items.size() + 1, 0.0);
   for (size_t q = 0; q < out.items.size(); ++q) {
     const int4 it = out.items[q];
     const int32_t nrows = (it.w >> 1) & 511, nnz = it.z - it.y;
-    const double c = (it.w & kBlockLongRow) ? kCostLong + kCostLongEdge * nnz
-                                            : kCostBlock + kCostEdge * nnz + kCostRow * nrows;
+    double c;
+    if (it.w & kBlockLongRow) {
+      c = kCostLong + kCostLongEdge * nnz;
+    } else {
+      int heavy = 0;
+      for (int32_t r = it.x; r < it.x + nrows; ++r) heavy += deg_of(row_ptr, r) > kSerialRow;
+      c = kCostBlock + kCostEdge * nnz + kCostRow * nrows + kCostHeavyRow * heavy;
+    }
     cum[q + 1] = cum[q] + c;
   }
   out.xcd_split.assign(kNumXcd + 1, 0);
