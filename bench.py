@@ -239,10 +239,17 @@ def load_traffic(workload, history, layout, schedule, scatter_steps=None):
     byt = lambda v: v["fetch_bytes"] + v["write_bytes"]
     kern = rec["kernels"]
     K = scatter_steps if scatter_steps and scatter_steps > 1 else 1
+
+    def targs(name):
+        """Template arguments of a kernel name, e.g. 'lev::scatter_kernel<unsigned short, 1, 4, 2>(...)'."""
+        inner = name.split("<", 1)[1].split(">", 1)[0] if "<" in name else ""
+        return [t.strip() for t in inner.split(",")]
+
     total = sum(byt(v) for k, v in kern.items() if "step_kernel" in k or "regroup_kernel" in k)
-    # the staged schedule's regroup of K steps at once
-    total += sum(byt(v) for k, v in kern.items() if "regroup_multi_kernel" in k and k.endswith(f", {K}>")) / K
-    scat = [(k, v) for k, v in kern.items() if "scatter_kernel" in k and (K == 1 or k.endswith(f", {K}>"))]
+    # the staged schedule's regroup of K steps at once: regroup_multi_kernel<H, K>
+    total += sum(byt(v) for k, v in kern.items() if "regroup_multi_kernel" in k and targs(k)[1:2] == [str(K)]) / K
+    # scatter_kernel<H, LAYOUT, K, ...>
+    scat = [(k, v) for k, v in kern.items() if "scatter_kernel" in k and (K == 1 or targs(k)[2:3] == [str(K)])]
     if scat:
         k, v = scat[0]
         per_block = byt(v)
