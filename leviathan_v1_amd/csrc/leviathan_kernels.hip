@@ -74,14 +74,15 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   __shared__ double s_red[2][kBlock / 64];
 
   const int tid = threadIdx.x;
-  // Row blocks to workgroups.  The direct and binned gathers read sources that
-  // neighbouring rows share (and, binned, the bucket of their rows), so each XCD
-  // takes one contiguous run of blocks and they meet in its L2.  The staged row
-  // pass gathers nothing: it streams, and the blocks are dealt round-robin so
-  // that the eight XCDs stream through neighbouring addresses (BA N = 10^7,
-  // uint16, staged/8: 0.603 -> 0.565 ms, against the contiguous runs).
+  // Row blocks to workgroups.  The binned row pass reads its rows' bucket, so
+  // each XCD takes one contiguous run of blocks and the bucket stays in its L2.
+  // The other row passes take the blocks in order, dealt round-robin, so that
+  // the eight XCDs stream through neighbouring addresses: BA N = 10^7, uint16,
+  // staged/8 0.603 -> 0.565 ms; the FP32 direct/8 drop-in 4.73 -> 4.66 ms; the
+  // 256-network Watts-Strogatz ensemble, direct/16, 1.26 -> 1.24 ms (its
+  // gathers' locality between neighbouring rows did not need the runs).
   int32_t b;
-  if constexpr (GATHER == 2) {
+  if constexpr (GATHER != 1) {
     b = (int32_t)blockIdx.x;
     if (b >= a.num_items) return;
   } else {
@@ -548,7 +549,7 @@ template <class H, int LAYOUT, int GATHER>
 static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
   int32_t longest = 0;
   for (int x = 0; x < kNumXcd; ++x) longest = std::max(longest, a.xcd_split[x + 1] - a.xcd_split[x]);
-  const dim3 grid((unsigned)(GATHER == 2 ? a.num_items : kNumXcd * longest)), block(kBlock);
+  const dim3 grid((unsigned)(GATHER != 1 ? a.num_items : kNumXcd * longest)), block(kBlock);
   switch (ept) {
     case 4:
       step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
