@@ -98,6 +98,7 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         return {
             "history": history, "layout": layout, "schedule": eng.schedule_name,
             "autotune_ms_per_step": tuned, "scatter_steps": eng.scatter_steps if chosen[0] != "direct" else None,
+            "row_steps": eng.row_steps,
             "scatter_steps_ms_per_step": k_ms,
             "seconds": el, "ms_per_step": 1e3 * el / steps,
             "edge_steps_per_s": wl.E * steps / el,
@@ -337,6 +338,10 @@ def main():
                "staged": "scatter_kernel+regroup_kernel+step_kernel"}[head["schedule"].split("/")[0]]
     if (head.get("scatter_steps") or 1) > 1:
         kernels = kernels.replace("scatter_kernel", f"(scatter_kernel+early_kernel)/{head['scatter_steps']}")
+    paired = head.get("row_steps") == 2 and (head["schedule"].startswith("direct") or
+                                             (head.get("scatter_steps") or 1) % 2 == 0)
+    if paired:
+        kernels = kernels.replace("step_kernel", "(pair_kernel+pair_finish_kernel)/2")
     achieved = alg / (step_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -367,7 +372,7 @@ def main():
         "data": "synthetic (seeded graph and initial state)",
         "config": {"workload": wl.name, "N": wl.N, "E": wl.E, "max_delay": wl.max_delay,
                    "history": hist, "layout": head["layout"], "schedule": head["schedule"],
-                   "scatter_steps": head.get("scatter_steps")},
+                   "scatter_steps": head.get("scatter_steps"), "row_steps": head.get("row_steps")},
         "hbm_gbs": head["hbm_gbs"],
         "plain_fp32": forms.get("f32"),
         "fp32_tuned": forms.get("f32-tuned"),

@@ -223,6 +223,23 @@ int leviathan_set_scatter_steps(LeviathanHandle* handle, int steps);
 int leviathan_get_scatter_steps(LeviathanHandle* handle);
 int leviathan_autotune_scatter_steps(LeviathanHandle* handle, float dt, int steps, double* ms_out);
 
+/* Paired row passes.  With steps = 2, leviathan_step_n runs consecutive steps
+ * t, t+1 (t even within a block) as one row pass over the edges and a light
+ * pass over the nodes: the row pass runs step t in full and step t+1's
+ * prediction, Hebbian update and coupling sum, so each weight is read and
+ * written once per two steps; after step t's r and gain update, the node pass
+ * integrates step t+1 with the new gain.  Legal when every delay is >= 1
+ * (step t+1 then reads no phase step t writes); used for the direct schedule
+ * and for two-phase schedules at an even number of steps per scatter pass
+ * (otherwise, and for leviathan_step and a remainder, steps run singly).
+ * Phases, weights, history, r and gains are bit-identical to single steps.
+ * The default is 2 for a graph without delay-0 edges and 1 otherwise; 2 is
+ * refused for a graph with delay-0 edges.
+ * leviathan_get_row_steps returns the setting or -1.
+ * Return 0 or -1.  (No counterpart in the reference.) */
+int leviathan_set_row_steps(LeviathanHandle* handle, int steps);
+int leviathan_get_row_steps(LeviathanHandle* handle);
+
 /* Input drive (reservoir): omega_i(t) = omega_i + drive_b[i] * u_t.
  * leviathan_set_drive uploads drive_b[N] (NULL switches the drive off);
  * leviathan_step_n_driven runs n steps with u_t = u[t]. */

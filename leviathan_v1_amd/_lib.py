@@ -86,6 +86,8 @@ SIGNATURES = {
     "leviathan_set_scatter_steps": (c_int, [vp, c_int]),
     "leviathan_get_scatter_steps": (c_int, [vp]),
     "leviathan_autotune_scatter_steps": (c_int, [vp, c_float, c_int, vp]),
+    "leviathan_set_row_steps": (c_int, [vp, c_int]),
+    "leviathan_get_row_steps": (c_int, [vp]),
     "leviathan_step_n_driven": (c_int, [vp, c_float, c_int, vp, vp]),
     "leviathan_features_setup": (c_int, [vp, vp, c_int, c_int]),
     "leviathan_step_n_features": (c_int, [vp, c_float, c_int, vp, vp]),

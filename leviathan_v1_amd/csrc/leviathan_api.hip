@@ -131,6 +131,7 @@ struct LeviathanHandle {
   int32_t N = 0;
   int64_t E = 0;
   int32_t max_delay = 0;
+  int32_t min_delay = 0;    // smallest edge delay (the paired row pass needs >= 1)
   int32_t n_slots = 0;      // S = max_delay + 2
   int64_t ring_pitch = 0;
   size_t ring_bytes = 0;    // the history ring's size
@@ -170,6 +171,8 @@ struct LeviathanHandle {
   uint32_t* d_early_pos = nullptr;  // binned plans' value slots of those edges
   int32_t early_cnt[lev::kMaxScatterSteps] = {};  // early_cnt[k] = edges with delay < k
   int scatter_steps = 1;   // consecutive steps one scatter pass gathers (two-phase schedules)
+  int row_steps = 1;       // consecutive steps one row pass covers: 2 = paired row passes (leviathan_set_row_steps)
+  float2* d_pair = nullptr;  // paired row pass: per node the second step's coupling sum and prediction error
   int hbuf_steps = 1;      // steps d_hbuf (and the staged schedule's d_hbuf2) have room for
   int4* d_sources = nullptr;  // phase-1 work items
   int32_t n_sources = 0;
@@ -276,7 +279,7 @@ void destroy(LeviathanHandle* h) {
                   h->d_omega,   h->d_omega_hat, h->d_ring,    h->d_partials, h->d_chunks, h->d_fin_done, h->d_ctrl,    h->d_step,
                   h->d_r_hist,  h->d_scratch,   h->d_drive_b, h->d_drive_u,  h->d_csc_ptr, h->d_delay_csc, h->d_dsrc,
                   h->d_pos,     h->d_hbuf,      h->d_hbuf2,   h->d_hpos,    h->d_rank_run, h->d_run_ptr, h->d_run_base, h->d_sources,  h->d_feat_nodes,
-                  h->d_early_src, h->d_early_delay, h->d_early_pos,
+                  h->d_early_src, h->d_early_delay, h->d_early_pos, h->d_pair,
                   h->d_X,       h->d_feat_base, h->d_cs_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -339,6 +342,8 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.hstride = 1;
   a.hpos = h->d_hpos;
   a.lpos = pl.d_lpos;
+  a.hbuf_next = nullptr;
+  a.pair = h->d_pair;
   a.dt = dt;
   a.eta = h->cfg.eta;
   a.lam = h->cfg.lam;
@@ -389,30 +394,61 @@ lev::ScatterArgs scatter_args(LeviathanHandle* h, const DevPlan& pl) {
   return sa;
 }
 
-// Steps one scatter pass gathers under the active plan (1 for the direct schedule).
+// Whether blocks of steps run paired row passes: asked for (row_steps = 2),
+// possible (no delay-0 edge, so step t+1 reads nothing step t writes), and the
+// block pairs up (the direct schedule, or an even number of steps per scatter
+// pass).
+bool paired(const LeviathanHandle* h) {
+  if (h->row_steps != 2 || h->min_delay < 1 || !h->d_pair) return false;
+  return h->plans[h->active].kind == LEV_SCHEDULE_DIRECT || h->scatter_steps % 2 == 0;
+}
+
+// Steps of one block under the active plan: the steps one scatter pass
+// gathers (two-phase schedules), or for the direct schedule 2 with paired row
+// passes and 1 without.
 int block_steps(const LeviathanHandle* h) {
-  return h->plans[h->active].kind == LEV_SCHEDULE_DIRECT ? 1 : h->scatter_steps;
+  if (h->plans[h->active].kind == LEV_SCHEDULE_DIRECT) return paired(h) ? 2 : 1;
+  return h->scatter_steps;
 }
 
 // Enqueue the step kernel(s) of the active plan, without the finalize, for
-// step k of a block of K consecutive steps whose values one scatter pass
-// gathers (K = 1: a step on its own).  Step 0 of a block runs the scatter
-// (and, staged, the regroup) for all K steps; step k > 0 first gathers its
-// edges with delay < k, straight into the array the row pass reads.
+// step k of a block of K consecutive steps (K = 1: a step on its own).
+// Two-phase schedules: step 0 of a block runs the scatter (and, staged, the
+// regroup) for all K steps; step k > 0 first gathers its edges with delay < k,
+// straight into the array the row pass reads.
+// Paired row passes (paired(h), K >= 2): an even step k runs the pair's row
+// pass, which leaves the odd step's integration to pair_finish_kernel, run
+// as step k + 1.  The early gathers of both steps of a pair run before it:
+// without delay-0 edges the odd step's read theta(t + k) at the latest, which
+// the previous pair wrote.
 // mid / mid2 (optional) are recorded after the scatter phase and after the regroup.
 hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t mid = nullptr,
                         hipEvent_t mid2 = nullptr) {
   const DevPlan& pl = h->plans[h->active];
-  if (pl.kind == LEV_SCHEDULE_DIRECT)
-    return lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, step_args(h, dt, pl), h->stream);
+  const bool pair = K >= 2 && paired(h);
+  const int gather = pl.kind == LEV_SCHEDULE_DIRECT ? 0 : pl.kind == LEV_SCHEDULE_STAGED ? 2 : 1;
+  hipError_t e = hipSuccess;
+  if (pair && k % 2 == 1) {  // the second step of a pair: its integration only
+    if (mid) e = hipEventRecord(mid, h->stream);
+    if (e == hipSuccess && mid2) e = hipEventRecord(mid2, h->stream);
+    if (e == hipSuccess) e = lev::launch_pair_finish(h->cfg.history, h->cfg.layout, step_args(h, dt, pl), h->stream);
+    return e;
+  }
+  if (pl.kind == LEV_SCHEDULE_DIRECT) {
+    const StepArgs a = step_args(h, dt, pl);
+    return pair ? lev::launch_pair(h->cfg.history, h->cfg.layout, pl.ept, 0, a, h->stream)
+                : lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, 0, a, h->stream);
+  }
   const bool staged = pl.kind == LEV_SCHEDULE_STAGED;
   const size_t hsize = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
   // the values step k's row pass reads: the regrouped array of step k (staged), or component k of
   // the bucketed slots, which hold the K steps' values side by side (binned)
-  void* hk = staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)k * h->E * hsize)
-                    : static_cast<void*>(static_cast<char*>(h->d_hbuf) + (size_t)k * hsize);
+  auto values_of = [&](int kk) {
+    return staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)kk * h->E * hsize)
+                  : static_cast<void*>(static_cast<char*>(h->d_hbuf) + (size_t)kk * hsize);
+  };
+  void* hk = values_of(k);
   const int32_t hstride = staged ? 1 : K;
-  hipError_t e = hipSuccess;
   if (k == 0) {
     lev::ScatterArgs sa = scatter_args(h, pl);
     sa.steps = K;
@@ -422,14 +458,17 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     ea.src = h->d_early_src;
     ea.pos = pl.d_early_pos;
     ea.delay = h->d_early_delay;
-    ea.count = h->early_cnt[k];
     ea.ring = h->d_ring;
     ea.ring_pitch = h->ring_pitch;
     ea.n_slots = h->n_slots;
-    ea.hbuf = hk;
     ea.hstride = hstride;
     ea.step = h->d_step;
-    e = lev::launch_early(h->cfg.history, h->cfg.layout, ea, h->stream);
+    for (int q = 0; q < (pair ? 2 : 1) && e == hipSuccess; ++q) {
+      ea.count = h->early_cnt[k + q];
+      ea.hbuf = values_of(k + q);
+      ea.step_off = q;
+      e = lev::launch_early(h->cfg.history, h->cfg.layout, ea, h->stream);
+    }
   }
   if (e == hipSuccess && mid) e = hipEventRecord(mid, h->stream);
   if (e == hipSuccess && staged && k == 0) {
@@ -449,7 +488,12 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     StepArgs a = step_args(h, dt, pl);
     a.hbuf = hk;
     a.hstride = hstride;
-    e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, staged ? 2 : 1, a, h->stream);
+    if (pair) {
+      a.hbuf_next = values_of(k + 1);
+      e = lev::launch_pair(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
+    } else {
+      e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
+    }
   }
   return e;
 }
@@ -679,6 +723,7 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   h->N = N;
   h->E = row_ptr[N];
   h->max_delay = max_delay;
+  h->min_delay = h->E > 0 ? *std::min_element(delays, delays + h->E) : max_delay;
   h->n_slots = max_delay + 2;
   h->n_nets = n_nets;
   auto fail = [&]() -> LeviathanHandle* {
@@ -830,8 +875,11 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       !dalloc(h, &h->d_fin_done, 0) ||
       !dalloc(h, &h->d_ctrl, (size_t)n_nets) ||
       !dalloc(h, &h->d_step, 1) || !dalloc(h, &h->d_r_hist, (size_t)lev::kRHistSteps * n_nets) ||
-      !dalloc(h, &h->d_scratch, (size_t)N) || !dalloc(h, &h->d_drive_u, (size_t)lev::kDriveSteps))
+      !dalloc(h, &h->d_scratch, (size_t)N) || !dalloc(h, &h->d_drive_u, (size_t)lev::kDriveSteps) ||
+      (h->min_delay >= 1 && !dalloc(h, &h->d_pair, (size_t)N)))
     return fail();
+  // paired row passes by default wherever the graph allows them (bit-identical to single steps)
+  h->row_steps = h->min_delay >= 1 ? 2 : 1;
   if (!hip_ok(hipHostMalloc((void**)&h->h_pinned, sizeof(float) * (size_t)lev::kRHistSteps * n_nets,
                             hipHostMallocDefault),
               "hipHostMalloc"))
@@ -1646,6 +1694,26 @@ int leviathan_set_scatter_steps(LeviathanHandle* h, int steps) {
 int leviathan_get_scatter_steps(LeviathanHandle* h) {
   if (!check_handle(h)) return -1;
   return h->scatter_steps;
+}
+
+int leviathan_set_row_steps(LeviathanHandle* h, int steps) {
+  if (!check_handle(h)) return -1;
+  if (steps != 1 && steps != 2) {
+    set_err("row steps must be 1 or 2 (got %d)", steps);
+    return -1;
+  }
+  if (steps == 2 && h->min_delay < 1) {
+    set_err("paired row passes need every delay >= 1; this graph has delay-0 edges");
+    return -1;
+  }
+  if (steps != h->row_steps) drop_graphs(h);  // captured graphs hold the old launch sequence
+  h->row_steps = steps;
+  return 0;
+}
+
+int leviathan_get_row_steps(LeviathanHandle* h) {
+  if (!check_handle(h)) return -1;
+  return h->row_steps;
 }
 
 int leviathan_autotune_scatter_steps(LeviathanHandle* h, float dt, int steps, double* ms_out) {

@@ -360,15 +360,17 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
 }
 
 // The values of the edges with delay < k for step t + k of a multi-step
-// scatter, read once step t + k - 1 has written theta(t + k - 1): one lane per
-// edge, in (delay, source) order so that the lanes of a load read one slot row
-// for ascending sources.
+// scatter: edge (j, d) reads theta_j(t + k - d), which step t + k - d - 1
+// writes.  The gather runs once step t + k - 1 is done, or (step_off = 1, a
+// graph without delay-0 edges) once step t + k - 2 is.  One lane per edge, in
+// (delay, source) order so that the lanes of a load read one slot row for
+// ascending sources.
 template <class H, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void early_kernel(EarlyArgs a) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.count) return;
   const int32_t S = a.n_slots;
-  const int32_t cur = (int32_t)(*a.step % S);
+  const int32_t cur = (int32_t)((*a.step + a.step_off) % S);
   int32_t slot = cur - (int32_t)a.delay[i];
   slot += (slot < 0) ? S : 0;
   const H* ring = reinterpret_cast<const H*>(a.ring);

@@ -69,6 +69,10 @@ struct StepArgs {
   const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
   int32_t hstride;        // binned schedule: elements per hbuf slot (the K interleaved steps of a scatter pass)
   const uint16_t* lpos;   // staged schedule: [E] position of each CSR edge's value in its block's range
+  // paired row pass (two steps per pass): the staged values of the second step, and per node the
+  // second step's coupling sum and prediction error, left for its integration
+  const void* hbuf_next;
+  float2* pair;
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
 
@@ -171,6 +175,8 @@ struct EarlyArgs {
   void* hbuf;               // the values the step's row pass reads (bucketed, or regrouped when staged)
   int32_t hstride;          // elements per slot of that array (binned: the pass's interleaved steps)
   const int64_t* step;
+  int32_t step_off;         // the values are those of step *step + step_off (1: gathered a step ahead,
+                            // for the second step of a paired row pass)
 };
 
 // The delayed values of a scatter pass's K steps for one edge, side by side
@@ -185,6 +191,11 @@ struct alignas(sizeof(H) * K) HVec {
 // gather: 0 = delayed phases read from the history ring (direct schedule),
 //         1 = read from the scatter kernel's bucketed array (binned schedule)
 hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s);
+// The paired row pass: steps t and t+1 of every row block but step t+1's
+// integration, which launch_pair_finish runs after step t's finalize.  Needs a
+// graph without delay-0 edges, a.pair, and (staged) a.hbuf_next.
+hipError_t launch_pair(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s);
+hipError_t launch_pair_finish(int history, int layout, const StepArgs& a, hipStream_t s);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
 hipError_t launch_fill_ring(int history, int layout, void* ring, int64_t pitch, int32_t n_slots,
                             const float* theta, int32_t n, hipStream_t s);

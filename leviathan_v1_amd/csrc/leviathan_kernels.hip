@@ -64,92 +64,196 @@ __device__ __forceinline__ Item unpack_item(int4 v) {
   return it;
 }
 
-template <class H, int LAYOUT, int EPT, int GATHER>
-__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
-  constexpr int CAP = kBlock * EPT;
-  __shared__ float s_theta[kBlock];
-  __shared__ float s_gate[kBlock];
-  __shared__ uint8_t s_erow[CAP];
-  __shared__ float s_contrib[CAP];
-  __shared__ double s_red[2][kBlock / 64];
+// Stream mode: the LDS byte map from each of the block's edges to its row.
+// Rows of up to kSerialRow edges are filled by their own lane, longer ones by
+// their whole wavefront, one after another (a lane looping over a long row
+// would hold its wavefront for the row's whole length).  `mine`: the lane owns
+// a row, whose edges are [rs, re) of the block.
+__device__ __forceinline__ void fill_edge_rows(uint8_t* s_erow, int tid, bool mine, bool heavy, int32_t rs,
+                                               int32_t re) {
+  const int lane = tid & 63;
+  if (mine && !heavy)
+    for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
+  for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+    const int src = __ffsll((unsigned long long)m) - 1;
+    const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+    for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
+  }
+}
 
+// Stream mode: each row's coupling sum of its edges' contributions in CSR
+// order: an FP64 sum by the row's lane, rounded once (two VALU instructions
+// per term where Neumaier's FP32 sum took about eight, in a loop the wave runs
+// for its longest row), or for a row of more than kSerialRow edges
+// lane-strided Neumaier partials combined by a fixed FP64 tree.
+__device__ __forceinline__ float row_coupling(const float* s_contrib, int tid, bool mine, bool heavy, int32_t rs,
+                                              int32_t re) {
+  const int lane = tid & 63;
+  float coupling = 0.0f;
+  if (mine && !heavy) {
+    double acc = 0.0;
+    for (int32_t q = rs; q < re; ++q) acc += (double)s_contrib[q];
+    coupling = (float)acc;
+  }
+  for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
+    const int src = __ffsll((unsigned long long)m) - 1;
+    const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
+    CompSum acc;
+    for (int32_t q = hs + lane; q < he; q += 64) acc.add(s_contrib[q]);
+    const double tot = wave_sum((double)acc.s + (double)acc.c);
+    if (lane == src) coupling = (float)tot;
+  }
+  return coupling;
+}
+
+// The block's FP64 partial of sum cos / sum sin of its new phases (lanes
+// without a row add zeros), summed in a fixed order: DPP wavefront sums, then
+// the four wavefronts in turn.
+__device__ __forceinline__ void store_block_partial(double my_c, double my_s, double (*s_red)[kBlock / 64],
+                                                    double2* out) {
   const int tid = threadIdx.x;
-  // Row blocks to workgroups.  The binned row pass reads its rows' bucket, so
-  // each XCD takes one contiguous run of blocks and the bucket stays in its L2.
-  // The other row passes take the blocks in order, dealt round-robin, so that
-  // the eight XCDs stream through neighbouring addresses: BA N = 10^7, uint16,
-  // staged/8 0.603 -> 0.565 ms; the FP32 direct/8 drop-in 4.73 -> 4.66 ms; the
-  // 256-network Watts-Strogatz ensemble, direct/16, 1.26 -> 1.24 ms (its
-  // gathers' locality between neighbouring rows did not need the runs).
-  int32_t b;
+  const double wc = wave_sum(my_c);
+  const double ws = wave_sum(my_s);
+  if ((tid & 63) == 0) {
+    s_red[0][tid >> 6] = wc;
+    s_red[1][tid >> 6] = ws;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double c = 0.0, s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      c += s_red[0][q];
+      s += s_red[1][q];
+    }
+    *out = make_double2(c, s);
+  }
+}
+
+// Long-row mode: the whole workgroup strides over one row's nnz edges from e0,
+// updating each weight and returning the lane's compensated partial of the
+// coupling.  The delayed values come from `hb`: the ring read at slot
+// cur - delay (GATHER 0), the bucketed array (GATHER 1, component already
+// added to hb) or the row's range of the staged array in CSR order (GATHER 2).
+// A lane visits the same edges whatever the step, so a second sweep over the
+// row reads back only the weights the lane itself stored.
+template <class H, int LAYOUT, int GATHER>
+__device__ __forceinline__ CompSum long_row_sweep(const StepArgs& a, const H* __restrict__ hb, int32_t cur,
+                                                  int32_t e0, int32_t nnz, float th_i, float gate, float dt) {
+  const int tid = threadIdx.x;
+  const int32_t S = a.n_slots;
+  const int64_t pitch = a.ring_pitch;
+  const float th_rev = th_i * kInvTwoPi;
+  CompSum acc;
+  for (int32_t base = 0; base < nnz; base += kBlock * 4) {
+    // unconditional loads (edge e0 for a lane past the row's end), as in stream mode
+    int32_t jj[4], dd[4];
+    float ww[4];
+    H hr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t le = base + k * kBlock + tid;
+      const int32_t e = e0 + (le < nnz ? le : 0);
+      if constexpr (GATHER == 0) {
+        jj[k] = a.col[e];
+        dd[k] = a.delay[e];
+      } else if constexpr (GATHER == 1) {
+        jj[k] = (int32_t)a.hpos[e];
+      } else {
+        jj[k] = e;
+      }
+      ww[k] = a.w[e];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if constexpr (GATHER == 0) {
+        int32_t slot = cur - dd[k];
+        slot += (slot < 0) ? S : 0;
+        hr[k] = hb[ring_index<LAYOUT>(pitch, jj[k], slot)];
+      } else if constexpr (GATHER == 1) {
+        hr[k] = hb[(int64_t)(uint32_t)jj[k] * a.hstride];
+      } else {
+        hr[k] = hb[(uint32_t)jj[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t le = base + k * kBlock + tid;
+      if (le < nnz) {
+        float c;
+        float wn;
+        if constexpr (sizeof(H) == 2)
+          wn = edge_update_rev((float)hr[k], th_rev, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        else
+          wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        acc.add(c);
+        a.w[e0 + le] = wn;
+      }
+    }
+  }
+  return acc;
+}
+
+// Long-row mode: the coupling of one row from its lanes' partials, by a fixed
+// FP64 tree (wavefront sums, then the four wavefronts in turn), returned to
+// lane 0.
+__device__ __forceinline__ float long_row_coupling(const CompSum& acc, double (*s_red)[kBlock / 64]) {
+  const int tid = threadIdx.x;
+  const double part = wave_sum((double)acc.s + (double)acc.c);
+  if ((tid & 63) == 0) s_red[0][tid >> 6] = part;
+  __syncthreads();
+  double tot = 0.0;
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) tot += s_red[0][q];
+  }
+  __syncthreads();  // s_red is reused
+  return (float)tot;
+}
+
+// Row block of workgroup blockIdx.x.  The binned row pass reads its rows'
+// bucket, so each XCD takes one contiguous run of blocks and the bucket stays
+// in its L2.  The other row passes take the blocks in order, dealt round-robin,
+// so that the eight XCDs stream through neighbouring addresses: BA N = 10^7,
+// uint16, staged/8 0.603 -> 0.565 ms; the FP32 direct/8 drop-in 4.73 -> 4.66
+// ms; the 256-network Watts-Strogatz ensemble, direct/16, 1.26 -> 1.24 ms (its
+// gathers' locality between neighbouring rows did not need the runs).
+// Returns -1 for a workgroup without a block.
+template <int GATHER>
+__device__ __forceinline__ int32_t row_block_of(const StepArgs& a) {
   if constexpr (GATHER != 1) {
-    b = (int32_t)blockIdx.x;
-    if (b >= a.num_items) return;
+    const int32_t b = (int32_t)blockIdx.x;
+    return b < a.num_items ? b : -1;
   } else {
     const int32_t x = blockIdx.x % kNumXcd;
-    b = a.xcd_split[x] + (int32_t)(blockIdx.x / kNumXcd);
-    if (b >= a.xcd_split[x + 1]) return;  // this XCD's run is shorter than the longest
+    const int32_t b = a.xcd_split[x] + (int32_t)(blockIdx.x / kNumXcd);
+    return b < a.xcd_split[x + 1] ? b : -1;  // this XCD's run is shorter than the longest
   }
-  const Item it = unpack_item(a.items[b]);  // carries its edge range: no row_ptr round trip first
-  if (it.nrows == 0) {  // padding item
-    if (threadIdx.x == 0) a.partials[b] = make_double2(0.0, 0.0);
-    return;
-  }
-  const int32_t r0 = it.r0, r1 = it.r0 + it.nrows, net = it.net;
-  const int64_t step = *a.step;
-  const int32_t S = a.n_slots;
-  const int32_t cur = (int32_t)(step % S);
-  const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
-  const float g = (float)a.ctrl[net].g;
-  const float dt = a.dt;
-  const int32_t e0 = it.e0;
-  const int32_t nnz = it.nnz;
-  const int32_t nrows = r1 - r0;
-  const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
-  const H* __restrict__ hbuf = reinterpret_cast<const H*>(a.hbuf);
-  const int64_t pitch = a.ring_pitch;
-  float u_t = 0.0f;
-  if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
+}
 
-  double my_c = 0.0, my_s = 0.0;
+// Stream mode: each lane's EPT edges of the block (edge le = k * kBlock + tid)
+// and the value each edge reads.  The loads are unconditional: a lane past the
+// block's edges reads edge 0 (every edge array holds at least one element) and
+// discards the value.  Straight-line code keeps all of a lane's loads in
+// flight at once; with a branch around each load the compiler waited for every
+// load before issuing the next.
+//
+// NV values per edge are read: those of the step (cur: the step's slot) and,
+// with NV = 2, those of the next step (the ring one slot later, the bucketed
+// slot's next component, or the next staged array a.hbuf_next).  The staged
+// schedule stages the block's values in LDS (s_v[v], as codes for uint16) and
+// reads edge e's at lpos[e]; the caller's barrier then separates the staging
+// from the reads, so this is done in two calls (STAGE then PICK).
+template <class H, int LAYOUT, int EPT, int GATHER, int NV>
+struct StreamEdges {
+  int32_t jj[EPT];  // column (GATHER 0), value slot (GATHER 1) or staged position (GATHER 2)
+  int32_t dd[EPT];
+  float ww[EPT];
+  float hh[NV][EPT];  // the delayed values, decoded (uint16: the code as a float)
 
-  if (!it.is_long) {
-    // ---------------- stream mode ----------------
-    RowState rs_;
-    int32_t rs = 0, re = 0;
-    float om = 0.0f;  // natural frequency (plus drive), loaded with the row's state: a load after
-                      // the last barrier would add a memory round trip to every row block
-    if (tid < nrows) {
-      const int32_t i = r0 + tid;
-      om = a.omega[i];
-      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
-      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
-      s_theta[tid] = sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th;  // uint16: revolutions (edge_update_rev)
-      s_gate[tid] = rs_.gate;
-      rs = a.row_ptr[i] - e0;
-      re = a.row_ptr[i + 1] - e0;
-    }
-    // Rows of up to kSerialRow edges are handled by their own lane, longer
-    // ones by their whole wavefront, one after another (a lane looping over a
-    // long row would hold its wavefront for the row's whole length).
-    const int lane = tid & 63;
-    const bool heavy = tid < nrows && re - rs > kSerialRow;
-    if (tid < nrows && !heavy)
-      for (int32_t q = rs; q < re; ++q) s_erow[q] = (uint8_t)tid;
-    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
-      const int src = __ffsll((unsigned long long)m) - 1;
-      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
-      for (int32_t q = hs + lane; q < he; q += 64) s_erow[q] = (uint8_t)((tid & ~63) + src);
-    }
-
-    // The loads are unconditional: a lane past the block's edges reads edge 0
-    // (every edge array holds at least one element) and discards the value.
-    // Straight-line code keeps all of a lane's loads in flight at once; with a
-    // branch around each load the compiler waited for every load before
-    // issuing the next.
-    int32_t jj[EPT];
-    int32_t dd[EPT];
-    float ww[EPT];
+  __device__ __forceinline__ void load(const StepArgs& a, int32_t e0, int32_t nnz, int32_t cur, float* const* s_v) {
+    const int tid = threadIdx.x;
+    const int32_t S = a.n_slots;
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;
@@ -165,54 +269,81 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       ww[k] = a.w[e];
     }
     if constexpr (GATHER == 2) {
-      // s_contrib holds the block's staged values until the edge pass below
-      H sv[EPT];
+      // the block's staged values go to LDS (s_v[v]) until the edge pass
+      H sv[NV][EPT];
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * kBlock + tid;
-        sv[k] = hbuf[le < nnz ? e0 + le : 0];
+      for (int v = 0; v < NV; ++v) {
+        const H* __restrict__ src = reinterpret_cast<const H*>(v == 0 ? a.hbuf : a.hbuf_next);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * kBlock + tid;
+          sv[v][k] = src[le < nnz ? e0 + le : 0];
+        }
       }
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * kBlock + tid;
-        if (le < nnz) s_contrib[le] = sizeof(H) == 2 ? (float)sv[k] : HistCodec<H>::decode(sv[k]);  // uint16: the code
-      }
-    }
-    __syncthreads();
-
-    float hh[EPT];
-    if constexpr (GATHER == 2) {
+      for (int v = 0; v < NV; ++v)
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int32_t le = k * kBlock + tid;
-        hh[k] = le < nnz ? s_contrib[jj[k]] : 0.0f;
-      }
+        for (int k = 0; k < EPT; ++k) {
+          const int32_t le = k * kBlock + tid;
+          if (le < nnz) s_v[v][le] = sizeof(H) == 2 ? (float)sv[v][k] : HistCodec<H>::decode(sv[v][k]);
+        }
     } else {
       // every lane's column, delay or slot is a valid one (edge 0 for a lane past the edges)
-      H hr[EPT];
+      const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
+      const H* __restrict__ hbuf = reinterpret_cast<const H*>(a.hbuf);
+      const int64_t pitch = a.ring_pitch;
+      H hr[NV][EPT];
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         if constexpr (GATHER == 0) {
           int32_t slot = cur - dd[k];
           slot += (slot < 0) ? S : 0;
-          hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
+          hr[0][k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
+          if constexpr (NV == 2) {
+            const int32_t slot2 = slot + 1 == S ? 0 : slot + 1;
+            hr[1][k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot2)];
+          }
         } else {
           // the bucketed array keeps a scatter pass's steps side by side: hstride values per slot
-          hr[k] = hbuf[(int64_t)(uint32_t)jj[k] * a.hstride];
+          const int64_t at = (int64_t)(uint32_t)jj[k] * a.hstride;
+          hr[0][k] = hbuf[at];
+          if constexpr (NV == 2) hr[1][k] = hbuf[at + 1];
         }
       }
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) hh[k] = sizeof(H) == 2 ? (float)hr[k] : HistCodec<H>::decode(hr[k]);
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) hh[v][k] = sizeof(H) == 2 ? (float)hr[v][k] : HistCodec<H>::decode(hr[v][k]);
     }
-    if constexpr (GATHER == 2) __syncthreads();  // every staged value is read before it is overwritten
+  }
+
+  // staged schedule, after the barrier that follows load(): each edge's values from LDS
+  __device__ __forceinline__ void pick(int32_t nnz, float* const* s_v) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t le = k * kBlock + tid;
+        hh[v][k] = le < nnz ? s_v[v][jj[k]] : 0.0f;
+      }
+  }
+
+  // The Hebbian update of the lane's edges with values hh[v]: each edge's
+  // contribution w_old * sin(delta) goes to s_contrib, and its new weight to
+  // ww (and, with store, to memory).  The edge arithmetic runs on every lane
+  // (a lane past the block's edges works on row 0 and a defined value and
+  // stores nothing), so that only the stores are predicated and the constants
+  // stay in registers across the unrolled edges.  s_theta holds each row's
+  // phase (revolutions for uint16 history) and s_gate its gate.
+  template <int V>
+  __device__ __forceinline__ void update(const StepArgs& a, int32_t e0, int32_t nnz, const uint8_t* s_erow,
+                                         const float* s_theta, const float* s_gate, float* s_contrib, bool store) {
+    const int tid = threadIdx.x;
     // the lane's weight stores address the weights by a 32-bit element index
     // from the kernel argument's base (E < 2^31), so no store needs 64-bit
     // address arithmetic of its own
     const uint32_t wlane = (uint32_t)(e0 + tid);
-    // The edge arithmetic runs on every lane (a lane past the block's edges
-    // works on row 0 and a defined value and stores nothing), so that only the
-    // two stores are predicated and the constants stay in registers across the
-    // unrolled edges.
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int32_t le = k * kBlock + tid;  // < CAP: the LDS read is in range
@@ -221,36 +352,86 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       float c;
       float wn;
       if constexpr (sizeof(H) == 2)
-        wn = edge_update_rev(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        wn = edge_update_rev(hh[V][k], s_theta[lr], s_gate[lr], ww[k], a.dt, a.eta, a.lam, a.w_min, a.w_max, &c);
       else
-        wn = edge_update(hh[k], s_theta[lr], s_gate[lr], ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+        wn = edge_update(hh[V][k], s_theta[lr], s_gate[lr], ww[k], a.dt, a.eta, a.lam, a.w_min, a.w_max, &c);
+      ww[k] = wn;
       if (live) {
         s_contrib[le] = c;  // coupling uses the pre-update weight (:154)
-        a.w[wlane + k * kBlock] = wn;
+        if (store) a.w[wlane + k * kBlock] = wn;
       }
     }
+  }
+};
+
+// One step of the row blocks (prep, gather, coupling, Hebbian update,
+// integration, history write, block partial of r).
+template <class H, int LAYOUT, int EPT, int GATHER>
+__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
+  constexpr int CAP = kBlock * EPT;
+  __shared__ float s_theta[kBlock];
+  __shared__ float s_gate[kBlock];
+  __shared__ uint8_t s_erow[CAP];
+  __shared__ float s_contrib[CAP];
+  __shared__ double s_red[2][kBlock / 64];
+
+  const int tid = threadIdx.x;
+  const int32_t b = row_block_of<GATHER>(a);
+  if (b < 0) return;
+  const Item it = unpack_item(a.items[b]);  // carries its edge range: no row_ptr round trip first
+  if (it.nrows == 0) {  // padding item
+    if (threadIdx.x == 0) a.partials[b] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int32_t r0 = it.r0, r1 = it.r0 + it.nrows, net = it.net;
+  const int64_t step = *a.step;
+  const int32_t S = a.n_slots;
+  const int32_t cur = (int32_t)(step % S);
+  const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
+  const float g = (float)a.ctrl[net].g;
+  const float dt = a.dt;
+  const int32_t e0 = it.e0;
+  const int32_t nnz = it.nnz;
+  const int32_t nrows = r1 - r0;
+  const int64_t pitch = a.ring_pitch;
+  float u_t = 0.0f;
+  if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
+
+  double my_c = 0.0, my_s = 0.0;
+
+  if (!it.is_long) {
+    // ---------------- stream mode ----------------
+    RowState rs_;
+    int32_t rs = 0, re = 0;
+    float om = 0.0f;  // natural frequency (plus drive), loaded with the row's state: a load after
+                      // the last barrier would add a memory round trip to every row block
+    const bool mine = tid < nrows;
+    if (mine) {
+      const int32_t i = r0 + tid;
+      om = a.omega[i];
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
+      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
+      s_theta[tid] = sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th;  // uint16: revolutions (edge_update_rev)
+      s_gate[tid] = rs_.gate;
+      rs = a.row_ptr[i] - e0;
+      re = a.row_ptr[i + 1] - e0;
+    }
+    const bool heavy = mine && re - rs > kSerialRow;
+    fill_edge_rows(s_erow, tid, mine, heavy, rs, re);
+
+    StreamEdges<H, LAYOUT, EPT, GATHER, 1> ed;
+    float* const s_v[1] = {s_contrib};  // staged schedule: s_contrib holds the values until the edge pass
+    ed.load(a, e0, nnz, cur, s_v);
+    __syncthreads();
+    if constexpr (GATHER == 2) {
+      ed.pick(nnz, s_v);
+      __syncthreads();  // every staged value is read before it is overwritten
+    }
+    ed.template update<0>(a, e0, nnz, s_erow, s_theta, s_gate, s_contrib, true);
     __syncthreads();
 
-    // Row sums in CSR order: an FP64 sum by the row's lane, rounded once (two
-    // VALU instructions per term where Neumaier's FP32 sum took about eight, in
-    // a loop the wave runs for its longest row), or for a long row lane-strided
-    // Neumaier partials combined by a fixed FP64 tree.
-    float coupling = 0.0f;
-    if (tid < nrows && !heavy) {
-      double acc = 0.0;
-      for (int32_t q = rs; q < re; ++q) acc += (double)s_contrib[q];
-      coupling = (float)acc;
-    }
-    for (uint64_t m = __ballot(heavy); m; m &= m - 1) {
-      const int src = __ffsll((unsigned long long)m) - 1;
-      const int32_t hs = __builtin_amdgcn_readlane(rs, src), he = __builtin_amdgcn_readlane(re, src);
-      CompSum acc;
-      for (int32_t q = hs + lane; q < he; q += 64) acc.add(s_contrib[q]);
-      const double tot = wave_sum((double)acc.s + (double)acc.c);
-      if (lane == src) coupling = (float)tot;
-    }
-
-    if (tid < nrows) {
+    const float coupling = row_coupling(s_contrib, tid, mine, heavy, rs, re);
+    if (mine) {
       const int32_t i = r0 + tid;
       const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
       a.theta[i] = x;
@@ -275,65 +456,11 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       s_gate[0] = rs_.gate;
     }
     __syncthreads();
-    const float th_i = s_theta[0];
-    const float th_rev = th_i * kInvTwoPi;
-    const float gate = s_gate[0];
-    CompSum acc;
-    for (int32_t base = 0; base < nnz; base += kBlock * 4) {
-      // unconditional loads (edge e0 for a lane past the row's end), as in stream mode
-      int32_t jj[4], dd[4];
-      float ww[4];
-      H hr[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int32_t le = base + k * kBlock + tid;
-        const int32_t e = e0 + (le < nnz ? le : 0);
-        if constexpr (GATHER == 0) {
-          jj[k] = a.col[e];
-          dd[k] = a.delay[e];
-        } else if constexpr (GATHER == 1) {
-          jj[k] = (int32_t)a.hpos[e];
-        } else {
-          jj[k] = e;
-        }
-        ww[k] = a.w[e];
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if constexpr (GATHER == 0) {
-          int32_t slot = cur - dd[k];
-          slot += (slot < 0) ? S : 0;
-          hr[k] = ring[ring_index<LAYOUT>(pitch, jj[k], slot)];
-        } else if constexpr (GATHER == 1) {
-          hr[k] = hbuf[(int64_t)(uint32_t)jj[k] * a.hstride];
-        } else {
-          hr[k] = hbuf[(uint32_t)jj[k]];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int32_t le = base + k * kBlock + tid;
-        if (le < nnz) {
-          float c;
-          float wn;
-          if constexpr (sizeof(H) == 2)
-            wn = edge_update_rev((float)hr[k], th_rev, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
-          else
-            wn = edge_update(HistCodec<H>::decode(hr[k]), th_i, gate, ww[k], dt, a.eta, a.lam, a.w_min, a.w_max, &c);
-          acc.add(c);
-          a.w[e0 + le] = wn;
-        }
-      }
-    }
-    // FP64 tree over the lanes' compensated partials (fixed order).
-    double part = wave_sum((double)acc.s + (double)acc.c);
-    if ((tid & 63) == 0) s_red[0][tid >> 6] = part;
-    __syncthreads();
+    const H* hb = reinterpret_cast<const H*>(GATHER == 0 ? a.ring : a.hbuf);
+    const CompSum acc = long_row_sweep<H, LAYOUT, GATHER>(a, hb, cur, e0, nnz, s_theta[0], s_gate[0], dt);
+    const float coupling = long_row_coupling(acc, s_red);
     if (tid == 0) {
-      double tot = 0.0;
-#pragma unroll
-      for (int q = 0; q < kBlock / 64; ++q) tot += s_red[0][q];
-      const float x = row_integrate(rs_, om, g, (float)tot, dt, a.alpha);
+      const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
       a.theta[i] = x;
       reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
       a.theta_hat[i] = rs_.thh_new;
@@ -343,26 +470,197 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
       my_c = (double)cx;
       my_s = (double)sx;
     }
-    __syncthreads();  // s_red is reused below
   }
+  store_block_partial(my_c, my_s, s_red, &a.partials[b]);
+}
 
-  // Block partial of sum cos / sum sin of the new phases, in FP64.
-  const double wc = wave_sum(my_c);
-  const double ws = wave_sum(my_s);
-  if ((tid & 63) == 0) {
-    s_red[0][tid >> 6] = wc;
-    s_red[1][tid >> 6] = ws;
+// The paired row pass: steps t and t + 1 of the row blocks in one pass, for
+// graphs without a delay-0 edge.  Step t + 1 reads theta_j(t + 1 - d) with
+// d >= 1, which exists before step t's row pass, and each row's own state
+// after step t is the workgroup's, so the workgroup runs step t in full and
+// then step t + 1's prep, Hebbian update and coupling sum, reading and writing
+// each weight once for the two steps.  Only step t + 1's integration waits:
+// it needs the gain g(t + 1), which the finalize of step t computes from the
+// order parameter of all of step t's new phases.  The pass leaves each row's
+// step t + 1 coupling sum and prediction error in a.pair, and
+// pair_finish_kernel integrates after that finalize.
+//
+// Every value is computed as the single-step kernel computes it, from the
+// same operands in the same order (the weights after step t stay in
+// registers; the coupling sum and the error are stored as the FP32 values the
+// integration reads), so phases, weights, history, r and gain are
+// bit-identical to two single steps.
+template <class H, int LAYOUT, int EPT, int GATHER>
+__global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
+  constexpr int CAP = kBlock * EPT;
+  __shared__ float s_theta[kBlock];
+  __shared__ float s_gate[kBlock];
+  __shared__ uint8_t s_erow[CAP];
+  __shared__ float s_contrib[CAP];
+  __shared__ float s_next[GATHER == 2 ? CAP : 1];  // staged schedule: step t + 1's values
+  __shared__ double s_red[2][kBlock / 64];
+
+  const int tid = threadIdx.x;
+  const int32_t b = row_block_of<GATHER>(a);
+  if (b < 0) return;
+  const Item it = unpack_item(a.items[b]);
+  if (it.nrows == 0) {  // padding item
+    if (threadIdx.x == 0) a.partials[b] = make_double2(0.0, 0.0);
+    return;
   }
-  __syncthreads();
-  if (tid == 0) {
-    double c = 0.0, s = 0.0;
-#pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) {
-      c += s_red[0][q];
-      s += s_red[1][q];
+  const int32_t r0 = it.r0, net = it.net;
+  const int64_t step = *a.step;
+  const int32_t S = a.n_slots;
+  const int32_t cur = (int32_t)(step % S);
+  const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
+  const float g = (float)a.ctrl[net].g;
+  const float dt = a.dt;
+  const int32_t e0 = it.e0;
+  const int32_t nnz = it.nnz;
+  const int32_t nrows = it.nrows;
+  const int64_t pitch = a.ring_pitch;
+  float u_t = 0.0f;
+  if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
+
+  double my_c = 0.0, my_s = 0.0;
+
+  if (!it.is_long) {
+    // ---------------- stream mode ----------------
+    RowState rs_;
+    int32_t rs = 0, re = 0;
+    float om = 0.0f;
+    const bool mine = tid < nrows;
+    if (mine) {
+      const int32_t i = r0 + tid;
+      om = a.omega[i];
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
+      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
+      s_theta[tid] = sizeof(H) == 2 ? rs_.th * kInvTwoPi : rs_.th;
+      s_gate[tid] = rs_.gate;
+      rs = a.row_ptr[i] - e0;
+      re = a.row_ptr[i + 1] - e0;
     }
-    a.partials[b] = make_double2(c, s);
+    const bool heavy = mine && re - rs > kSerialRow;
+    fill_edge_rows(s_erow, tid, mine, heavy, rs, re);
+
+    StreamEdges<H, LAYOUT, EPT, GATHER, 2> ed;
+    float* const s_v[2] = {s_contrib, s_next};
+    ed.load(a, e0, nnz, cur, s_v);
+    __syncthreads();
+    if constexpr (GATHER == 2) {
+      ed.pick(nnz, s_v);
+      __syncthreads();  // every staged value is read before s_contrib is overwritten
+    }
+
+    // step t: the new weights stay in the lanes' registers
+    ed.template update<0>(a, e0, nnz, s_erow, s_theta, s_gate, s_contrib, false);
+    __syncthreads();
+    const float coupling = row_coupling(s_contrib, tid, mine, heavy, rs, re);
+    RowState rs2;
+    if (mine) {
+      const int32_t i = r0 + tid;
+      const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
+      a.theta[i] = x;
+      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      float sx, cx;
+      sincos_fast(x, &sx, &cx);
+      my_c = (double)cx;
+      my_s = (double)sx;
+      // step t + 1's prep from the state step t leaves (every edge of step t
+      // read s_theta and s_gate before the barrier ahead of the row sums)
+      rs2 = row_prep(x, rs_.thh_new, rs_.omh_new, dt, a.kappa_w, a.gamma);
+      a.theta_hat[i] = rs2.thh_new;
+      a.omega_hat[i] = rs2.omh_new;
+      s_theta[tid] = sizeof(H) == 2 ? rs2.th * kInvTwoPi : rs2.th;
+      s_gate[tid] = rs2.gate;
+    }
+    __syncthreads();  // every row sum has read s_contrib
+
+    // step t + 1 from the weights step t left
+    ed.template update<1>(a, e0, nnz, s_erow, s_theta, s_gate, s_contrib, true);
+    __syncthreads();
+    const float coupling2 = row_coupling(s_contrib, tid, mine, heavy, rs, re);
+    if (mine) a.pair[r0 + tid] = make_float2(coupling2, rs2.eps);
+  } else {
+    // ---------------- long-row mode: one row, whole workgroup, two sweeps ----------------
+    const int32_t i = r0;
+    RowState rs_, rs2;
+    float om = 0.0f;
+    if (tid == 0) {
+      om = a.omega[i];
+      if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
+      rs_ = row_prep(a.theta[i], a.theta_hat[i], a.omega_hat[i], dt, a.kappa_w, a.gamma);
+      s_theta[0] = rs_.th;
+      s_gate[0] = rs_.gate;
+    }
+    __syncthreads();
+    const H* hb = reinterpret_cast<const H*>(GATHER == 0 ? a.ring : a.hbuf);
+    const CompSum acc = long_row_sweep<H, LAYOUT, GATHER>(a, hb, cur, e0, nnz, s_theta[0], s_gate[0], dt);
+    const float coupling = long_row_coupling(acc, s_red);
+    if (tid == 0) {
+      const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
+      a.theta[i] = x;
+      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      float sx, cx;
+      sincos_fast(x, &sx, &cx);
+      my_c = (double)cx;
+      my_s = (double)sx;
+      rs2 = row_prep(x, rs_.thh_new, rs_.omh_new, dt, a.kappa_w, a.gamma);
+      a.theta_hat[i] = rs2.thh_new;
+      a.omega_hat[i] = rs2.omh_new;
+      s_theta[0] = rs2.th;
+      s_gate[0] = rs2.gate;
+    }
+    __syncthreads();
+    // step t + 1: the ring one slot later, the bucketed slot's next component, or the next staged array
+    const H* hb2 = GATHER == 0   ? reinterpret_cast<const H*>(a.ring)
+                   : GATHER == 1 ? reinterpret_cast<const H*>(a.hbuf) + 1
+                                 : reinterpret_cast<const H*>(a.hbuf_next);
+    const CompSum acc2 = long_row_sweep<H, LAYOUT, GATHER>(a, hb2, nxt, e0, nnz, s_theta[0], s_gate[0], dt);
+    const float coupling2 = long_row_coupling(acc2, s_red);
+    if (tid == 0) a.pair[i] = make_float2(coupling2, rs2.eps);
   }
+  store_block_partial(my_c, my_s, s_red, &a.partials[b]);
+}
+
+// Second half of a paired row pass: step t + 1's integration of every row,
+// after the finalize of step t has set g(t + 1) and advanced the step counter,
+// from the coupling sum and prediction error pair_kernel left.  One workgroup
+// per row block, with the rows on the lanes the single-step kernel puts them
+// on, so the block partials of r are the single-step kernel's to the bit.
+template <class H, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void pair_finish_kernel(StepArgs a) {
+  __shared__ double s_red[2][kBlock / 64];
+  const int tid = threadIdx.x;
+  const int32_t b = (int32_t)blockIdx.x;
+  if (b >= a.num_items) return;
+  const Item it = unpack_item(a.items[b]);
+  if (it.nrows == 0) {
+    if (tid == 0) a.partials[b] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int64_t step = *a.step;
+  const int32_t S = a.n_slots;
+  const int32_t cur = (int32_t)(step % S);
+  const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
+  double my_c = 0.0, my_s = 0.0;
+  if (tid < (it.is_long ? 1 : it.nrows)) {
+    const int32_t i = it.r0 + tid;
+    float om = a.omega[i];
+    if (a.drive_b) om = fmaf(a.drive_b[i], a.drive_u[step % kDriveSteps], om);
+    const float2 ce = a.pair[i];
+    RowState rs;
+    rs.th = a.theta[i];
+    rs.eps = ce.y;
+    const float x = row_integrate(rs, om, (float)a.ctrl[it.net].g, ce.x, a.dt, a.alpha);
+    a.theta[i] = x;
+    reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(a.ring_pitch, i, nxt)] = HistCodec<H>::encode(x);
+    float sx, cx;
+    sincos_fast(x, &sx, &cx);
+    my_c = (double)cx;
+    my_s = (double)sx;
+  }
+  store_block_partial(my_c, my_s, s_red, &a.partials[b]);
 }
 
 // Sum of kChunkPartials consecutive partials per workgroup, in a fixed order.
@@ -545,20 +843,23 @@ __global__ void read_slot_kernel(const H* ring, int64_t pitch, int32_t slot, flo
 // launchers
 // ---------------------------------------------------------------------------
 
-template <class H, int LAYOUT, int GATHER>
+template <class H, int LAYOUT, int GATHER, bool PAIR>
 static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
   int32_t longest = 0;
   for (int x = 0; x < kNumXcd; ++x) longest = std::max(longest, a.xcd_split[x + 1] - a.xcd_split[x]);
   const dim3 grid((unsigned)(GATHER != 1 ? a.num_items : kNumXcd * longest)), block(kBlock);
   switch (ept) {
     case 4:
-      step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
+      if constexpr (PAIR) pair_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
+      else step_kernel<H, LAYOUT, 4, GATHER><<<grid, block, 0, s>>>(a);
       break;
     case 8:
-      step_kernel<H, LAYOUT, 8, GATHER><<<grid, block, 0, s>>>(a);
+      if constexpr (PAIR) pair_kernel<H, LAYOUT, 8, GATHER><<<grid, block, 0, s>>>(a);
+      else step_kernel<H, LAYOUT, 8, GATHER><<<grid, block, 0, s>>>(a);
       break;
     case 16:
-      step_kernel<H, LAYOUT, 16, GATHER><<<grid, block, 0, s>>>(a);
+      if constexpr (PAIR) pair_kernel<H, LAYOUT, 16, GATHER><<<grid, block, 0, s>>>(a);
+      else step_kernel<H, LAYOUT, 16, GATHER><<<grid, block, 0, s>>>(a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -566,26 +867,56 @@ static hipError_t launch_step_t(int ept, const StepArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class H, int LAYOUT>
+template <class H, int LAYOUT, bool PAIR>
 static hipError_t launch_step_g(int ept, int gather, const StepArgs& a, hipStream_t s) {
   switch (gather) {
     case 0:
-      return launch_step_t<H, LAYOUT, 0>(ept, a, s);
+      return launch_step_t<H, LAYOUT, 0, PAIR>(ept, a, s);
     case 1:
-      return launch_step_t<H, LAYOUT, 1>(ept, a, s);
+      return launch_step_t<H, LAYOUT, 1, PAIR>(ept, a, s);
     case 2:
-      return launch_step_t<H, LAYOUT, 2>(ept, a, s);
+      return launch_step_t<H, LAYOUT, 2, PAIR>(ept, a, s);
     default:
       return hipErrorInvalidValue;
   }
 }
 
+template <bool PAIR>
+static hipError_t launch_step_p(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
+  if (history == 0)
+    return layout == 0 ? launch_step_g<float, 0, PAIR>(ept, gather, a, s)
+                       : launch_step_g<float, 1, PAIR>(ept, gather, a, s);
+  return layout == 0 ? launch_step_g<uint16_t, 0, PAIR>(ept, gather, a, s)
+                     : launch_step_g<uint16_t, 1, PAIR>(ept, gather, a, s);
+}
+
 hipError_t launch_step(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
   if (a.num_items <= 0) return hipSuccess;
+  return launch_step_p<false>(history, layout, ept, gather, a, s);
+}
+
+hipError_t launch_pair(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
+  if (a.num_items <= 0) return hipSuccess;
+  if (!a.pair || (gather == 2 && !a.hbuf_next)) return hipErrorInvalidValue;
+  return launch_step_p<true>(history, layout, ept, gather, a, s);
+}
+
+hipError_t launch_pair_finish(int history, int layout, const StepArgs& a, hipStream_t s) {
+  if (a.num_items <= 0) return hipSuccess;
+  if (!a.pair) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)a.num_items), block(kBlock);
   if (history == 0) {
-    return layout == 0 ? launch_step_g<float, 0>(ept, gather, a, s) : launch_step_g<float, 1>(ept, gather, a, s);
+    if (layout == 0)
+      pair_finish_kernel<float, 0><<<grid, block, 0, s>>>(a);
+    else
+      pair_finish_kernel<float, 1><<<grid, block, 0, s>>>(a);
+  } else {
+    if (layout == 0)
+      pair_finish_kernel<uint16_t, 0><<<grid, block, 0, s>>>(a);
+    else
+      pair_finish_kernel<uint16_t, 1><<<grid, block, 0, s>>>(a);
   }
-  return layout == 0 ? launch_step_g<uint16_t, 0>(ept, gather, a, s) : launch_step_g<uint16_t, 1>(ept, gather, a, s);
+  return hipGetLastError();
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {

@@ -415,6 +415,20 @@ class LeviathanEngine:
             _lib.check("autotune_scatter_steps")
         return {k: float(v) for k, v in zip((1, 2, 4), out) if v == v}
 
+    @property
+    def row_steps(self) -> int:
+        """Consecutive steps of step_n one row pass covers: 2 (paired row passes) or 1."""
+        k = _lib.lib().leviathan_get_row_steps(self._h)
+        if k < 0:
+            _lib.check("get_row_steps")
+        return k
+
+    def set_row_steps(self, steps: int) -> None:
+        """1 or 2 (paired row passes, refused for a graph with delay-0 edges); results are
+        bit-identical either way."""
+        if _lib.lib().leviathan_set_row_steps(self._h, int(steps)) != 0:
+            _lib.check("set_row_steps")
+
     def autotune_all(self, dt: float, steps: int = 8) -> dict:
         """The plans, then the steps per scatter pass of the fastest two-phase plan, then the plans
         again (the two-phase ones at that many steps per pass).  Each timing starts from the current

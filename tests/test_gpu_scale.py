@@ -5,9 +5,9 @@ up to ~10^4), delays uniform in [1, 50], FP32 and uint16 history.
     taken from the engine's exported state (the bar of the other GPU tests), and the
     step's r within 1e-6 of the FP64 order parameter of the phases it wrote;
   * the benchmark's form (uint16 history, staged schedule with 8 edges per lane, two steps
-    per scatter pass): each step of a two-step block against an FP64 oracle step from the
-    engine's state, and the block bit-identical to the same two steps taken one scatter pass
-    each;
+    per scatter pass, a paired row pass): each step of a two-step block against an FP64
+    oracle step from the engine's state, and the block bit-identical to the same two steps
+    taken one scatter pass and one row pass each;
   * 2,000 steps of the uint16 form and of the plain FP32 drop-in against the FP64 oracle's
     2,000 steps of the same inputs (the committed fixture tests/golden/ba10m_fp64.npz,
     written by tests/golden/make_golden.py with the OpenMP FP64 oracle): windowed
@@ -74,7 +74,8 @@ def test_ba10m_one_step_vs_fp64_oracle(ba10m, history, schedule):
 
 def test_ba10m_two_step_block_vs_fp64_oracle(ba10m):
     """The benchmark's form takes its steps in blocks: one scatter pass and one regroup pass
-    gather the delayed phases of two steps, then two row passes run.  From the state at a block
+    gather the delayed phases of two steps, then one paired row pass runs the first step and
+    the second step's edge work, and a node pass the second step's integration.  From the state at a block
     boundary, each step of the block is compared with an FP64 oracle step taken from the
     engine's own state before it (1e-5 relative): the first from the block engine's exported
     state, the second from the state of an engine that took the first step alone, which is
@@ -87,7 +88,7 @@ def test_ba10m_two_step_block_vs_fp64_oracle(ba10m):
     a = make(ba10m, history="u16", schedule="staged", edges_per_thread=8)
     b = make(ba10m, history="u16", schedule="staged", edges_per_thread=8)
     a.set_scatter_steps(2)
-    assert (a.scatter_steps, b.scatter_steps) == (2, 1)
+    assert (a.scatter_steps, b.scatter_steps) == (2, 1) and a.row_steps == 2
     np.testing.assert_array_equal(a.step_n(DT, 20), b.step_n(DT, 20))
     o1 = oracle_from_engine(a, ba10m, parallel=True)
     o1.step(DT)
@@ -97,7 +98,7 @@ def test_ba10m_two_step_block_vs_fp64_oracle(ba10m):
     o2.step(DT)
     o1.step(DT)  # free-running second FP64 step, for reference
     b.step(DT)
-    ra = a.step_n(DT, 2)  # one block: scatter + regroup of two steps, two row passes
+    ra = a.step_n(DT, 2)  # one block: scatter + regroup of two steps, a paired row pass, a node pass
     th = a.get_theta()
     e2t, e2w = theta_rel_err(th, o2.theta), w_rel_err(a.get_weights(), o2.w)
     say("step 1: theta", e1t, "w", e1w, "| step 2 from the engine's state: theta", e2t, "w", e2w,
