@@ -117,6 +117,10 @@ struct DevPlan {
   int32_t n_rg_chunks = 0;
   // two-phase plans: each early edge's slot in the array the row pass reads (multi-step scatter)
   uint32_t* d_early_pos = nullptr;
+  // staged plans: the early edges of each row block, gathered by the second pair of a block's
+  // paired row passes (block_early_edges)
+  int32_t* d_pe_ptr = nullptr;
+  int2* d_pe = nullptr;
   bool owns_pos = false;
 };
 
@@ -285,8 +289,8 @@ void destroy(LeviathanHandle* h) {
     if (p) (void)hipFree(p);
   for (auto& pl : h->plans) {
     if (pl.owns_pos) {
-      void* pp[] = {pl.d_pos,   pl.d_rank_run,  pl.d_run_ptr, pl.d_run_base,
-                    pl.d_lpos,  pl.d_rg_chunks, pl.d_rg_rank,  pl.d_rg_runs, pl.d_early_pos};
+      void* pp[] = {pl.d_pos,   pl.d_rank_run,  pl.d_run_ptr, pl.d_run_base, pl.d_lpos, pl.d_rg_chunks,
+                    pl.d_rg_rank, pl.d_rg_runs, pl.d_early_pos, pl.d_pe_ptr, pl.d_pe};
       for (void* p : pp)
         if (p) (void)hipFree(p);
     }
@@ -344,6 +348,9 @@ StepArgs step_args(LeviathanHandle* h, float dt, const DevPlan& pl) {
   a.lpos = pl.d_lpos;
   a.hbuf_next = nullptr;
   a.pair = h->d_pair;
+  a.early_ptr = nullptr;
+  a.early = nullptr;
+  a.early_k = 0;
   a.dt = dt;
   a.eta = h->cfg.eta;
   a.lam = h->cfg.lam;
@@ -449,11 +456,13 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
   };
   void* hk = values_of(k);
   const int32_t hstride = staged ? 1 : K;
+  // the second pair of a staged block gathers its early edges in the row pass itself
+  const bool pair_early = pair && staged && k > 0;
   if (k == 0) {
     lev::ScatterArgs sa = scatter_args(h, pl);
     sa.steps = K;
     e = lev::launch_scatter(h->cfg.history, h->cfg.layout, sa, h->stream);
-  } else {
+  } else if (!pair_early) {
     lev::EarlyArgs ea{};
     ea.src = h->d_early_src;
     ea.pos = pl.d_early_pos;
@@ -490,6 +499,11 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     a.hstride = hstride;
     if (pair) {
       a.hbuf_next = values_of(k + 1);
+      if (pair_early) {
+        a.early_ptr = pl.d_pe_ptr;
+        a.early = pl.d_pe;
+        a.early_k = k;
+      }
       e = lev::launch_pair(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
     } else {
       e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
@@ -785,6 +799,8 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   std::vector<int32_t> early_src, early_cnt;
   std::vector<uint8_t> early_delay;
   std::vector<uint32_t> early_csr, bin_early_pos, st_early_pos[3];
+  std::vector<int32_t> st_pe_ptr[3];
+  std::vector<int2> st_pe[3];
   const bool want_binned = cfg.schedule == LEV_SCHEDULE_BINNED || cfg.schedule == LEV_SCHEDULE_AUTO;
   // the scatter kernel's launcher makes the same choice (leviathan_binned.hip)
   const bool scatter_window = lev::scatter_uses_window(cfg.layout, h->n_slots, hsize);
@@ -837,6 +853,10 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
       // early edges are gathered straight into the regrouped array: their staged slots
       st_early_pos[q].resize(early_csr.size());
       for (size_t i = 0; i < early_csr.size(); ++i) st_early_pos[q][i] = hpos[early_csr[i]];
+      if (!lev::block_early_edges(hp[q], early_src, early_delay, early_csr, hpos, st_pe_ptr[q], st_pe[q], perr)) {
+        set_err("%s", perr.c_str());
+        return fail();
+      }
       DevPlan& ps = h->plans[kPlanStaged + q];
       ps.built = true;
       ps.kind = LEV_SCHEDULE_STAGED;
@@ -962,7 +982,13 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
             !upload_runs(h, st_rank[q], st_runs[q], &ps.d_rank_run, &ps.d_run_ptr, &ps.d_run_base,
                          scatter_window ? &hsc : nullptr) ||
             !dalloc(h, &ps.d_rg_chunks, rg.chunks.size()) || !dalloc(h, &ps.d_rg_rank, E) ||
-            !dalloc(h, &ps.d_rg_runs, rg.runs.size()) || !dalloc(h, &ps.d_early_pos, ne))
+            !dalloc(h, &ps.d_rg_runs, rg.runs.size()) || !dalloc(h, &ps.d_early_pos, ne) ||
+            !dalloc(h, &ps.d_pe_ptr, st_pe_ptr[q].size()) || !dalloc(h, &ps.d_pe, st_pe[q].size()))
+          return fail();
+        if (!hip_ok(hipMemcpy(ps.d_pe_ptr, st_pe_ptr[q].data(), sizeof(int32_t) * st_pe_ptr[q].size(), H2D),
+                    "copy block early edges") ||
+            (!st_pe[q].empty() &&
+             !hip_ok(hipMemcpy(ps.d_pe, st_pe[q].data(), sizeof(int2) * st_pe[q].size(), H2D), "copy block early edges")))
           return fail();
         if (ne && !hip_ok(hipMemcpy(ps.d_early_pos, st_early_pos[q].data(), sizeof(uint32_t) * ne, H2D),
                           "copy early"))

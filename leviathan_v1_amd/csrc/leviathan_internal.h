@@ -23,6 +23,7 @@ __host__ __device__ inline int4 pack_item(int32_t r0, int32_t e0, int32_t e1, in
                                           bool is_long) {
   return make_int4(r0, e0, e1, (net << 10) | (nrows << 1) | (is_long ? kBlockLongRow : 0));
 }
+constexpr int kFinishBlocks = 8;     // row blocks per workgroup of the paired pass's node pass
 constexpr int kSerialRow = 128;      // stream blocks: longer rows are filled and summed by a wavefront
 // Estimated row-pass cost of a block (ns of workgroup lifetime), for balancing the XCDs.
 constexpr double kCostBlock = 9860.0, kCostEdge = 7.42, kCostRow = 11.9, kCostHeavyRow = 300.0;
@@ -73,6 +74,13 @@ struct StepArgs {
   // second step's coupling sum and prediction error, left for its integration
   const void* hbuf_next;
   float2* pair;
+  // staged schedule, the second pair of a multi-step block (early_k = its first step's index k > 0):
+  // each row block's early edges, early[early_ptr[b] .. early_ptr[b + 1]) = {source, position in the
+  // block's range << 2 | delay}; the pass gathers step k's value of those with delay < k and step
+  // k + 1's of all of them from the ring
+  const int32_t* early_ptr;
+  const int2* early;
+  int32_t early_k;
   float dt, eta, lam, gamma, alpha, kappa_w, w_min, w_max;
 };
 

@@ -136,9 +136,11 @@ __device__ __forceinline__ void store_block_partial(double my_c, double my_s, do
 // cur - delay (GATHER 0), the bucketed array (GATHER 1, component already
 // added to hb) or the row's range of the staged array in CSR order (GATHER 2).
 // A lane visits the same edges whatever the step, so a second sweep over the
-// row reads back only the weights the lane itself stored.
+// row reads back only the weights the lane itself stored.  (hb is not
+// __restrict__: the paired pass writes a long row's early values into the
+// staged arrays before it sweeps them.)
 template <class H, int LAYOUT, int GATHER>
-__device__ __forceinline__ CompSum long_row_sweep(const StepArgs& a, const H* __restrict__ hb, int32_t cur,
+__device__ __forceinline__ CompSum long_row_sweep(const StepArgs& a, const H* hb, int32_t cur,
                                                   int32_t e0, int32_t nnz, float th_i, float gate, float dt) {
   const int tid = threadIdx.x;
   const int32_t S = a.n_slots;
@@ -474,6 +476,21 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
   store_block_partial(my_c, my_s, s_red, &a.partials[b]);
 }
 
+// Staged schedule, second pair (steps k, k + 1, k > 0) of a multi-step block:
+// an early edge's values of both steps straight from the ring, at slots
+// cur - delay and cur + 1 - delay (the step counter is at step k).  Step k's
+// value replaces the scatter's only for delay < k; step k + 1's always does.
+template <class H, int LAYOUT>
+__device__ __forceinline__ void early_pair_values(const StepArgs& a, int2 rec, int32_t cur, H* v0, H* v1) {
+  const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
+  const int32_t S = a.n_slots;
+  int32_t slot = cur - (rec.y & 3);
+  slot += (slot < 0) ? S : 0;
+  const int32_t slot2 = slot + 1 == S ? 0 : slot + 1;
+  *v0 = ring[ring_index<LAYOUT>(a.ring_pitch, rec.x, slot)];
+  *v1 = ring[ring_index<LAYOUT>(a.ring_pitch, rec.x, slot2)];
+}
+
 // The paired row pass: steps t and t + 1 of the row blocks in one pass, for
 // graphs without a delay-0 edge.  Step t + 1 reads theta_j(t + 1 - d) with
 // d >= 1, which exists before step t's row pass, and each row's own state
@@ -484,6 +501,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 // order parameter of all of step t's new phases.  The pass leaves each row's
 // step t + 1 coupling sum and prediction error in a.pair, and
 // pair_finish_kernel integrates after that finalize.
+//
+// In the second pair of a multi-step block of the staged schedule, the pass
+// also gathers its block's early edges (a.early_k > 0): the block patches
+// their values in its LDS copy of the staged values (stream mode) or in its
+// range of the staged arrays before sweeping it (long-row mode), where
+// early_kernel would have written them to the staged arrays beforehand.
 //
 // Every value is computed as the single-step kernel computes it, from the
 // same operands in the same order (the weights after step t stay in
@@ -543,11 +566,39 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     const bool heavy = mine && re - rs > kSerialRow;
     fill_edge_rows(s_erow, tid, mine, heavy, rs, re);
 
+    // staged schedule, second pair of a block: this lane's first early edge, fetched with the other loads
+    int32_t eq = 0, eq1 = 0;
+    int2 erec = make_int2(0, -1);
+    H ev0{}, ev1{};
+    if constexpr (GATHER == 2) {
+      if (a.early_k > 0) {
+        eq = a.early_ptr[b] + tid;
+        eq1 = a.early_ptr[b + 1];
+        if (eq < eq1) {
+          erec = a.early[eq];
+          early_pair_values<H, LAYOUT>(a, erec, cur, &ev0, &ev1);
+        }
+      }
+    }
     StreamEdges<H, LAYOUT, EPT, GATHER, 2> ed;
     float* const s_v[2] = {s_contrib, s_next};
     ed.load(a, e0, nnz, cur, s_v);
     __syncthreads();
     if constexpr (GATHER == 2) {
+      if (a.early_k > 0) {  // uniform: patch the staged copies once every staging store is done
+        auto as_lds = [](H v) { return sizeof(H) == 2 ? (float)v : HistCodec<H>::decode(v); };
+        while (eq < eq1) {  // the fetched record, then the lane's next one (a block with > kBlock)
+          const int32_t lp = erec.y >> 2;
+          if ((erec.y & 3) < a.early_k) s_contrib[lp] = as_lds(ev0);
+          s_next[lp] = as_lds(ev1);
+          eq += kBlock;
+          if (eq < eq1) {
+            erec = a.early[eq];
+            early_pair_values<H, LAYOUT>(a, erec, cur, &ev0, &ev1);
+          }
+        }
+        __syncthreads();
+      }
       ed.pick(nnz, s_v);
       __syncthreads();  // every staged value is read before s_contrib is overwritten
     }
@@ -586,6 +637,22 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     const int32_t i = r0;
     RowState rs_, rs2;
     float om = 0.0f;
+    if constexpr (GATHER == 2) {
+      // second pair of a block: the row's early values go to its range of the staged arrays (CSR
+      // order), which the sweeps read after the barrier below
+      if (a.early_k > 0) {
+        H* const h0 = const_cast<H*>(reinterpret_cast<const H*>(a.hbuf));
+        H* const h1 = const_cast<H*>(reinterpret_cast<const H*>(a.hbuf_next));
+        for (int32_t q = a.early_ptr[b] + tid; q < a.early_ptr[b + 1]; q += kBlock) {
+          const int2 rec = a.early[q];
+          H v0, v1;
+          early_pair_values<H, LAYOUT>(a, rec, cur, &v0, &v1);
+          const int32_t slot = e0 + (rec.y >> 2);
+          if ((rec.y & 3) < a.early_k) h0[slot] = v0;
+          h1[slot] = v1;
+        }
+      }
+    }
     if (tid == 0) {
       om = a.omega[i];
       if (a.drive_b) om = fmaf(a.drive_b[i], u_t, om);
@@ -625,42 +692,81 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
 
 // Second half of a paired row pass: step t + 1's integration of every row,
 // after the finalize of step t has set g(t + 1) and advanced the step counter,
-// from the coupling sum and prediction error pair_kernel left.  One workgroup
-// per row block, with the rows on the lanes the single-step kernel puts them
-// on, so the block partials of r are the single-step kernel's to the bit.
+// from the coupling sum and prediction error pair_kernel left.  A workgroup
+// takes kFinishBlocks row blocks, with each block's rows on the lanes the
+// single-step kernel puts them on (a long row on lane 0), so each block
+// partial of r is the single-step kernel's to the bit.  All its loads are
+// issued first.  One block per workgroup (10^5 workgroups of ~100 rows at BA
+// N = 10^7) was bound by workgroup dispatch: 0.104 ms for 0.2 GB.
 template <class H, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void pair_finish_kernel(StepArgs a) {
-  __shared__ double s_red[2][kBlock / 64];
+  constexpr int NB = kFinishBlocks;
+  __shared__ double s_red[NB][2][kBlock / 64];
   const int tid = threadIdx.x;
-  const int32_t b = (int32_t)blockIdx.x;
-  if (b >= a.num_items) return;
-  const Item it = unpack_item(a.items[b]);
-  if (it.nrows == 0) {
-    if (tid == 0) a.partials[b] = make_double2(0.0, 0.0);
-    return;
-  }
+  const int32_t b0 = (int32_t)blockIdx.x * NB;
   const int64_t step = *a.step;
   const int32_t S = a.n_slots;
   const int32_t cur = (int32_t)(step % S);
   const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
-  double my_c = 0.0, my_s = 0.0;
-  if (tid < (it.is_long ? 1 : it.nrows)) {
-    const int32_t i = it.r0 + tid;
-    float om = a.omega[i];
-    if (a.drive_b) om = fmaf(a.drive_b[i], a.drive_u[step % kDriveSteps], om);
-    const float2 ce = a.pair[i];
-    RowState rs;
-    rs.th = a.theta[i];
-    rs.eps = ce.y;
-    const float x = row_integrate(rs, om, (float)a.ctrl[it.net].g, ce.x, a.dt, a.alpha);
-    a.theta[i] = x;
-    reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(a.ring_pitch, i, nxt)] = HistCodec<H>::encode(x);
-    float sx, cx;
-    sincos_fast(x, &sx, &cx);
-    my_c = (double)cx;
-    my_s = (double)sx;
+  const float u_t = a.drive_b ? a.drive_u[step % kDriveSteps] : 0.0f;
+  int32_t row[NB];
+  float th[NB], om[NB], g[NB];
+  float2 ce[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    row[j] = -1;
+    if (b0 + j < a.num_items) {
+      const Item it = unpack_item(a.items[b0 + j]);
+      const int32_t rows = it.nrows == 0 ? 0 : it.is_long ? 1 : it.nrows;
+      if (tid < rows) {
+        const int32_t i = it.r0 + tid;
+        row[j] = i;
+        th[j] = a.theta[i];
+        om[j] = a.omega[i];
+        if (a.drive_b) om[j] = fmaf(a.drive_b[i], u_t, om[j]);
+        ce[j] = a.pair[i];
+        g[j] = (float)a.ctrl[it.net].g;
+      }
+    }
   }
-  store_block_partial(my_c, my_s, s_red, &a.partials[b]);
+  double my_c[NB], my_s[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    my_c[j] = 0.0;
+    my_s[j] = 0.0;
+    if (row[j] >= 0) {
+      RowState rs;
+      rs.th = th[j];
+      rs.eps = ce[j].y;
+      const float x = row_integrate(rs, om[j], g[j], ce[j].x, a.dt, a.alpha);
+      a.theta[row[j]] = x;
+      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(a.ring_pitch, row[j], nxt)] = HistCodec<H>::encode(x);
+      float sx, cx;
+      sincos_fast(x, &sx, &cx);
+      my_c[j] = (double)cx;
+      my_s[j] = (double)sx;
+    }
+  }
+  // each block's partial as store_block_partial forms it: wavefront sums, then the four in turn
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const double wc = wave_sum(my_c[j]);
+    const double ws = wave_sum(my_s[j]);
+    if ((tid & 63) == 0) {
+      s_red[j][0][tid >> 6] = wc;
+      s_red[j][1][tid >> 6] = ws;
+    }
+  }
+  __syncthreads();
+  if (tid < NB && b0 + tid < a.num_items) {
+    double c = 0.0, s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      c += s_red[tid][0][q];
+      s += s_red[tid][1][q];
+    }
+    a.partials[b0 + tid] = make_double2(c, s);
+  }
 }
 
 // Sum of kChunkPartials consecutive partials per workgroup, in a fixed order.
@@ -904,7 +1010,7 @@ hipError_t launch_pair(int history, int layout, int ept, int gather, const StepA
 hipError_t launch_pair_finish(int history, int layout, const StepArgs& a, hipStream_t s) {
   if (a.num_items <= 0) return hipSuccess;
   if (!a.pair) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)a.num_items), block(kBlock);
+  const dim3 grid((unsigned)((a.num_items + kFinishBlocks - 1) / kFinishBlocks)), block(kBlock);
   if (history == 0) {
     if (layout == 0)
       pair_finish_kernel<float, 0><<<grid, block, 0, s>>>(a);

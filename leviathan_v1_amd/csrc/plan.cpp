@@ -436,4 +436,43 @@ void early_edges(int N, const int* row_ptr, const int* col_idx, const uint8_t* d
   }
 }
 
+bool block_early_edges(const HostPlan& plan, const std::vector<int32_t>& src, const std::vector<uint8_t>& delay,
+                       const std::vector<uint32_t>& csr, const std::vector<uint32_t>& staged_hpos,
+                       std::vector<int32_t>& ptr, std::vector<int2>& rec, std::string& err) {
+  const size_t n_items = plan.items.size();
+  // items by first edge (long rows come first in launch order), to find each edge's item
+  std::vector<int32_t> by_e0(n_items);
+  for (size_t q = 0; q < n_items; ++q) by_e0[q] = (int32_t)q;
+  std::sort(by_e0.begin(), by_e0.end(), [&](int32_t x, int32_t y) { return plan.items[x].y < plan.items[y].y; });
+  std::vector<int32_t> starts(n_items);
+  for (size_t k = 0; k < n_items; ++k) starts[k] = plan.items[by_e0[k]].y;
+  std::vector<int32_t> item_of(csr.size(), -1);
+  for (size_t r = 0; r < csr.size(); ++r) {
+    const int32_t e = (int32_t)csr[r];
+    size_t k = (size_t)(std::upper_bound(starts.begin(), starts.end(), e) - starts.begin());
+    // padding items hold no edges: step back over empty ranges to the item that holds e
+    while (k > 0 && !(plan.items[by_e0[k - 1]].y <= e && e < plan.items[by_e0[k - 1]].z)) --k;
+    if (k == 0 || delay[r] > 3) {
+      err = "internal error: an early edge outside every row block, or a delay above 3";
+      return false;
+    }
+    item_of[r] = by_e0[k - 1];
+  }
+  ptr.assign(n_items + 1, 0);
+  for (int32_t q : item_of) ++ptr[(size_t)q + 1];
+  for (size_t q = 0; q < n_items; ++q) ptr[q + 1] += ptr[q];
+  rec.assign(csr.size(), make_int2(0, 0));
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (size_t r = 0; r < csr.size(); ++r) {
+    const int4 it = plan.items[item_of[r]];
+    const int64_t local = (int64_t)staged_hpos[csr[r]] - it.y;
+    if (local < 0 || local >= ((int64_t)1 << 29)) {
+      err = "internal error: an early edge's staged slot lies outside its row block";
+      return false;
+    }
+    rec[fill[item_of[r]]++] = make_int2(src[r], (int32_t)(local << 2) | delay[r]);
+  }
+  return true;
+}
+
 }  // namespace lev

@@ -125,4 +125,13 @@ void early_edges(int N, const int* row_ptr, const int* col_idx, const uint8_t* d
                  std::vector<int32_t>& src, std::vector<uint8_t>& delay, std::vector<uint32_t>& csr,
                  std::vector<int32_t>& cnt);
 
+// Paired row passes of the staged schedule: each row block's early edges
+// (the records of early_edges), so that the second pair of a multi-step block
+// gathers their values itself.  ptr[q] .. ptr[q + 1] are item q's records
+// {source, (staged slot - the item's first edge) << 2 | delay}, in the order
+// of the early list; delays must be below 4.
+bool block_early_edges(const HostPlan& plan, const std::vector<int32_t>& src, const std::vector<uint8_t>& delay,
+                       const std::vector<uint32_t>& csr, const std::vector<uint32_t>& staged_hpos,
+                       std::vector<int32_t>& ptr, std::vector<int2>& rec, std::string& err);
+
 }  // namespace lev
