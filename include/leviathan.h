@@ -41,9 +41,11 @@ typedef struct LeviathanHandle LeviathanHandle;
 /* leviathan_kernel.cu:270-389.  Deep-copies the incoming-edge CSR graph
  * (row_ptr[N+1], col_idx[E], delays[E], E = row_ptr[N]) and the initial state
  * (weights[E], theta[N], theta_hat[N], omega[N]; omega_hat starts as omega)
- * to the GPU, with the parameter defaults of :281-297, FP32 history in the
- * slot-major layout (the faster one for every schedule measured; the layout
- * is internal and does not change any result).  Returns NULL on failure. */
+ * to the GPU, with the parameter defaults of :281-297 and FP32 history.  The
+ * history layout is chosen automatically (LEV_LAYOUT_AUTO: node-major when
+ * every delay is >= 1, so paired row passes read both steps' delayed phases of
+ * a node from one line; slot-major otherwise); the layout is internal and does
+ * not change any result.  Returns NULL on failure. */
 LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr,
                                 const int* h_col_idx, const uint8_t* h_delays,
                                 const float* h_weights, const float* h_theta,
@@ -89,7 +91,9 @@ enum {
 };
 enum {
   LEV_LAYOUT_NODE_MAJOR = 0, /* ring[node][slot] */
-  LEV_LAYOUT_SLOT_MAJOR = 1  /* ring[slot][node] */
+  LEV_LAYOUT_SLOT_MAJOR = 1, /* ring[slot][node] */
+  LEV_LAYOUT_AUTO = 2        /* the default: node-major for the direct schedule on a graph without
+                                delay-0 edges (paired row passes), slot-major otherwise */
 };
 
 typedef struct {
@@ -239,6 +243,11 @@ int leviathan_autotune_scatter_steps(LeviathanHandle* handle, float dt, int step
  * Return 0 or -1.  (No counterpart in the reference.) */
 int leviathan_set_row_steps(LeviathanHandle* handle, int steps);
 int leviathan_get_row_steps(LeviathanHandle* handle);
+
+/* The history layout a handle holds (LEV_LAYOUT_NODE_MAJOR or
+ * LEV_LAYOUT_SLOT_MAJOR; LEV_LAYOUT_AUTO is resolved when the handle is
+ * created), or -1.  (No counterpart in the reference.) */
+int leviathan_get_layout(LeviathanHandle* handle);
 
 /* Input drive (reservoir): omega_i(t) = omega_i + drive_b[i] * u_t.
  * leviathan_set_drive uploads drive_b[N] (NULL switches the drive off);

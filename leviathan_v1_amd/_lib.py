@@ -41,7 +41,7 @@ class LeviathanConfig(ctypes.Structure):
 
 
 HISTORY = {"f32": 0, "u16": 1}
-LAYOUT = {"node": 0, "slot": 1}
+LAYOUT = {"node": 0, "slot": 1, "auto": 2}
 SCHEDULE = {"direct": 0, "binned": 1, "auto": 2, "staged": 3}
 PLAN_NAMES = [f"{k}/{e}" for k in ("direct", "binned", "staged") for e in (4, 8, 16)]
 
@@ -88,6 +88,7 @@ SIGNATURES = {
     "leviathan_autotune_scatter_steps": (c_int, [vp, c_float, c_int, vp]),
     "leviathan_set_row_steps": (c_int, [vp, c_int]),
     "leviathan_get_row_steps": (c_int, [vp]),
+    "leviathan_get_layout": (c_int, [vp]),
     "leviathan_step_n_driven": (c_int, [vp, c_float, c_int, vp, vp]),
     "leviathan_features_setup": (c_int, [vp, vp, c_int, c_int]),
     "leviathan_step_n_features": (c_int, [vp, c_float, c_int, vp, vp]),

@@ -47,7 +47,9 @@ class LeviathanEngine:
 
     Keyword extensions:
       history: "f32" (default) or "u16" (exact-wrap uint16 angles)
-      layout:  "slot" (ring[slot][node], default) or "node" (ring[node][slot])
+      layout:  "auto" (default: "node" for the direct schedule on a graph without delay-0
+               edges, where paired row passes read a node's two slots from one line; "slot"
+               otherwise), "slot" (ring[slot][node]) or "node" (ring[node][slot])
       edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
       schedule: "direct" (one fused kernel, per-edge gather), "binned" (source-
                ordered scatter into destination buckets, then the fused row
@@ -61,7 +63,7 @@ class LeviathanEngine:
     """
 
     def __init__(self, N, max_delay, row_ptr, col_idx, delays, weights, theta, theta_hat, omega, *,
-                 history: str = "f32", layout: str = "slot", edges_per_thread: int = 8, graph_chunk: int = 100,
+                 history: str = "f32", layout: str = "auto", edges_per_thread: int = 8, graph_chunk: int = 100,
                  schedule: str = "direct", bucket_edges: int = 0,
                  params: Optional[dict] = None, n_nets: int = 1, net_offsets: Optional[Sequence[int]] = None,
                  g0: Optional[Sequence[float]] = None, controller: Optional[Sequence[bool]] = None):
@@ -82,7 +84,7 @@ class LeviathanEngine:
         self.n_nets = int(n_nets)
         # identifies the graph in checkpoints (restore refuses another graph's state)
         self.graph_digest = _checkpoint.graph_digest(row_ptr, col_idx, delays)
-        extended = (history != "f32" or layout != "slot" or edges_per_thread != 8 or graph_chunk != 100
+        extended = (history != "f32" or layout != "auto" or edges_per_thread != 8 or graph_chunk != 100
                     or schedule != "direct" or bucket_edges
                     or params or n_nets != 1 or net_offsets is not None or g0 is not None
                     or controller is not None)
@@ -121,7 +123,8 @@ class LeviathanEngine:
             raise RuntimeError("leviathan_init failed: " + _lib.last_error())
         self._h = ctypes.c_void_p(h)
         self.history = history
-        self.layout = layout
+        # the layout the handle holds ("auto" resolved)
+        self.layout = {v: k for k, v in _lib.LAYOUT.items()}[lib.leviathan_get_layout(self._h)]
 
     # ------------------------------------------------------------------ reference surface
     def step(self, dt: float) -> float:
