@@ -64,12 +64,14 @@ def make_workload(name, n_override=None):
 
 
 def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", bucket_edges=0, profile_steps=20,
-             scatter_steps="auto"):
+             scatter_steps="auto", row_steps=None):
     from leviathan_v1_amd import LeviathanEngine
 
     t0 = time.perf_counter()
     eng = LeviathanEngine(*wl.engine_args(), history=history, layout=layout, edges_per_thread=ept,
                           schedule=schedule, bucket_edges=bucket_edges)
+    if row_steps:
+        eng.set_row_steps(row_steps)
     log(f"{history}: engine ready in {time.perf_counter() - t0:.1f} s")
     try:
         tuned = k_ms = None
@@ -96,7 +98,7 @@ def run_form(wl, history, steps, warmup, layout="node", ept=8, schedule="auto", 
         theta = eng.get_theta()
         ok = bool(np.isfinite(theta).all() and np.isfinite(r).all() and 0 <= r.min() and r.max() <= 1)
         return {
-            "history": history, "layout": layout, "schedule": eng.schedule_name,
+            "history": history, "layout": eng.layout, "schedule": eng.schedule_name,
             "autotune_ms_per_step": tuned, "scatter_steps": eng.scatter_steps if chosen[0] != "direct" else None,
             "row_steps": eng.row_steps,
             "scatter_steps_ms_per_step": k_ms,
@@ -216,23 +218,24 @@ def copy_calibration(nbytes: int, reps: int = 3) -> None:
     log(f"copy calibration: {reps} x {nbytes} bytes device-to-device")
 
 
-def traffic_key(workload, history, layout, schedule, scatter_steps=None):
+def traffic_key(workload, history, layout, schedule, scatter_steps=None, paired=False):
     """Key of a workload's record in profiles/traffic.json."""
     key = f"{workload}:{history}:{schedule}" if layout == "node" else f"{workload}:{history}:{layout}:{schedule}"
-    return key + (f":k{scatter_steps}" if scatter_steps and scatter_steps > 1 else "")
+    return key + (f":k{scatter_steps}" if scatter_steps and scatter_steps > 1 else "") + (":p2" if paired else "")
 
 
-def load_traffic(workload, history, layout, schedule, scatter_steps=None):
+def load_traffic(workload, history, layout, schedule, scatter_steps=None, paired=False):
     """HBM bytes per step of the step's kernels, from the rocprofv3 counter runs
     summarised in profiles/traffic.json (None if that workload was not profiled).
     With K steps per scatter pass, a step carries 1/K of the scatter pass and of
-    its block's early-edge gathers."""
+    its block's early-edge gathers; with paired row passes, half of a pair's row
+    pass and node pass."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            rec = json.load(f).get(traffic_key(workload, history, layout, schedule, scatter_steps))
+            rec = json.load(f).get(traffic_key(workload, history, layout, schedule, scatter_steps, paired))
     except (OSError, ValueError):
         return None
     if not rec:
@@ -247,6 +250,8 @@ def load_traffic(workload, history, layout, schedule, scatter_steps=None):
         return [t.strip() for t in inner.split(",")]
 
     total = sum(byt(v) for k, v in kern.items() if "step_kernel" in k or "regroup_kernel" in k)
+    # paired row passes: the pair's row pass and node pass, once per two steps
+    total += sum(byt(v) for k, v in kern.items() if "pair_kernel" in k or "pair_finish_kernel" in k) / 2
     # the staged schedule's regroup of K steps at once: regroup_multi_kernel<H, K>
     total += sum(byt(v) for k, v in kern.items() if "regroup_multi_kernel" in k and targs(k)[1:2] == [str(K)]) / K
     # scatter_kernel<H, LAYOUT, K, ...>
@@ -269,14 +274,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ba10m", choices=["ba10m", "ws100k", "ensemble", "reservoir"])
     ap.add_argument("--n", type=int, default=None, help="override the node count (testing only)")
-    ap.add_argument("--layout", default="slot", choices=["node", "slot"],
-                    help="history ring of the f32-tuned and u16 forms (the plain drop-in uses leviathan_init's, "
-                         "slot-major)")
+    ap.add_argument("--layout", default="slot", choices=["node", "slot", "auto"],
+                    help="history ring of the f32-tuned and u16 forms (the plain drop-in uses leviathan_init's "
+                         "automatic choice)")
     ap.add_argument("--ept", type=int, default=8)
     ap.add_argument("--schedule", default="auto", choices=["auto", "direct", "binned", "staged"])
     ap.add_argument("--bucket-edges", type=int, default=0)
     ap.add_argument("--scatter-steps", default="auto", choices=["auto", "1", "2", "4"],
                     help="steps per scatter pass of the two-phase schedules (default: timed and chosen)")
+    ap.add_argument("--row-steps", type=int, default=None, choices=[1, 2],
+                    help="steps per row pass of the f32-tuned and u16 forms (default: the engine's, 2 where the "
+                         "graph allows paired row passes)")
     ap.add_argument("--static-reservoir", action="store_true", help="reservoir: no plasticity, no controller")
     ap.add_argument("--sweep-alphas", default="", help="reservoir: comma-separated ridge parameters to refit with")
     ap.add_argument("--sweep-washouts", default="", help="reservoir: comma-separated washout lengths to refit with")
@@ -321,11 +329,11 @@ def main():
     for f in args.forms.split(","):
         if f == "f32":
             # the plain drop-in exactly as leviathan_init configures it
-            forms[f] = run_form(wl, "f32", args.steps, args.warmup, "slot", 8, "direct")
+            forms[f] = run_form(wl, "f32", args.steps, args.warmup, "auto", 8, "direct")
         else:
             hist = "f32" if f == "f32-tuned" else f
             forms[f] = run_form(wl, hist, args.steps, args.warmup, args.layout, args.ept, args.schedule,
-                                args.bucket_edges, scatter_steps=args.scatter_steps)
+                                args.bucket_edges, scatter_steps=args.scatter_steps, row_steps=args.row_steps)
         log(f, json.dumps(forms[f]))
     head = forms.get("u16") or forms.get("f32-tuned") or forms["f32"]
     hist = head["history"]
@@ -336,10 +344,14 @@ def main():
     step_ms = head["step_kernel_ms"] + head["scatter_kernel_ms"] + head["regroup_kernel_ms"]
     kernels = {"direct": "step_kernel", "binned": "scatter_kernel+step_kernel",
                "staged": "scatter_kernel+regroup_kernel+step_kernel"}[head["schedule"].split("/")[0]]
-    if (head.get("scatter_steps") or 1) > 1:
-        kernels = kernels.replace("scatter_kernel", f"(scatter_kernel+early_kernel)/{head['scatter_steps']}")
     paired = head.get("row_steps") == 2 and (head["schedule"].startswith("direct") or
                                              (head.get("scatter_steps") or 1) % 2 == 0)
+    K = head.get("scatter_steps") or 1
+    if K > 1:
+        # the early-edge gathers: their own kernel, or (staged, paired) the second pair's row pass
+        early = "" if paired and head["schedule"].startswith("staged") else "+early_kernel"
+        kernels = kernels.replace("scatter_kernel", f"(scatter_kernel{early})/{K}")
+        kernels = kernels.replace("regroup_kernel", f"regroup_multi_kernel/{K}")
     if paired:
         kernels = kernels.replace("step_kernel", "(pair_kernel+pair_finish_kernel)/2")
     achieved = alg / (step_ms * 1e-3) / 1e9
@@ -384,7 +396,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(args.workload, hist, head["layout"], head["schedule"],
-                                             head.get("scatter_steps")),
+                                             head.get("scatter_steps"), paired),
                      "kernel": kernels, "kernel_ms": step_ms, "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": cpu,
         **subs,

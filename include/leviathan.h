@@ -42,10 +42,8 @@ typedef struct LeviathanHandle LeviathanHandle;
  * (row_ptr[N+1], col_idx[E], delays[E], E = row_ptr[N]) and the initial state
  * (weights[E], theta[N], theta_hat[N], omega[N]; omega_hat starts as omega)
  * to the GPU, with the parameter defaults of :281-297 and FP32 history.  The
- * history layout is chosen automatically (LEV_LAYOUT_AUTO: node-major when
- * every delay is >= 1, so paired row passes read both steps' delayed phases of
- * a node from one line; slot-major otherwise); the layout is internal and does
- * not change any result.  Returns NULL on failure. */
+ * history layout is chosen automatically (LEV_LAYOUT_AUTO); it is internal and
+ * does not change any result.  Returns NULL on failure. */
 LeviathanHandle* leviathan_init(int N, int max_delay, const int* h_row_ptr,
                                 const int* h_col_idx, const uint8_t* h_delays,
                                 const float* h_weights, const float* h_theta,
@@ -93,7 +91,10 @@ enum {
   LEV_LAYOUT_NODE_MAJOR = 0, /* ring[node][slot] */
   LEV_LAYOUT_SLOT_MAJOR = 1, /* ring[slot][node] */
   LEV_LAYOUT_AUTO = 2        /* the default: node-major for the direct schedule on a graph without
-                                delay-0 edges (paired row passes), slot-major otherwise */
+                                delay-0 edges (paired row passes then read a node's two delayed
+                                phases from one line) and with little index locality (under half
+                                of the sources within 256 nodes of their rows); slot-major
+                                otherwise */
 };
 
 typedef struct {

@@ -657,6 +657,21 @@ bool validate_inputs(int N, int max_delay, const int* row_ptr, const int* col_id
   return true;
 }
 
+// Share of (a sample of up to ~2^20) edges whose source lies within 256 nodes of its row: index
+// locality, as in ring lattices (Watts-Strogatz: ~0.9), against ~0 for Barabasi-Albert graphs.
+double near_source_share(int N, const int* row_ptr, const int* col_idx) {
+  const int64_t E = row_ptr[N];
+  if (E == 0) return 1.0;
+  const int64_t stride = std::max<int64_t>(1, E >> 20);
+  int64_t seen = 0, near = 0;
+  for (int i = 0; i < N; ++i)
+    for (int64_t e = (row_ptr[i] + stride - 1) / stride * stride; e < row_ptr[i + 1]; e += stride) {
+      ++seen;
+      near += std::abs((int64_t)col_idx[e] - i) < 256;
+    }
+  return seen ? (double)near / (double)seen : 1.0;
+}
+
 LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, const int* row_ptr,
                         const int* col_idx, const uint8_t* delays, const float* weights, const float* theta,
                         const float* theta_hat, const float* omega, int n_nets, const int* net_offsets,
@@ -740,13 +755,19 @@ LeviathanHandle* create(const LeviathanConfig* cfg_in, int N, int max_delay, con
   h->min_delay = h->E > 0 ? *std::min_element(delays, delays + h->E) : max_delay;
   h->n_slots = max_delay + 2;
   // Automatic ring layout: node-major for a direct-schedule handle whose graph allows paired row
-  // passes (a node's slots t - d and t + 1 - d then share a line, one gather for both steps:
-  // FP32 direct/8 at BA N = 10^7 2.95 ms per step against 4.30 on the slot-major ring), slot-major
-  // otherwise (the two-phase schedules' scatter reads slot rows; unpaired, the node-major ring's
-  // partial-line history writes made it the slower one: 5.24 against 4.87 ms)
-  if (cfg.layout == LEV_LAYOUT_AUTO)
-    cfg.layout = cfg.schedule == LEV_SCHEDULE_DIRECT && h->min_delay >= 1 ? LEV_LAYOUT_NODE_MAJOR
-                                                                           : LEV_LAYOUT_SLOT_MAJOR;
+  // passes and has little index locality, slot-major otherwise.  With pairs, a node's slots
+  // t - d and t + 1 - d share a line of the node-major ring, one gather for both steps; where
+  // gathers miss the caches that halves their lines (FP32 direct/8 at Barabasi-Albert N = 10^7:
+  // 2.95 ms per step against 4.30 on the slot-major ring).  Where most sources lie near their
+  // rows the gathers hit the L2 on either ring and the node-major ring's partial-line history
+  // writes cost more (256 Watts-Strogatz networks, direct/4-16: 1.14 against 0.90 ms); so do
+  // they without pairs (5.24 against 4.87 ms at BA N = 10^7).  The two-phase schedules' scatter
+  // reads slot rows.
+  if (cfg.layout == LEV_LAYOUT_AUTO) {
+    bool node = cfg.schedule == LEV_SCHEDULE_DIRECT && h->min_delay >= 1;
+    if (node) node = near_source_share(N, row_ptr, col_idx) < 0.5;
+    cfg.layout = node ? LEV_LAYOUT_NODE_MAJOR : LEV_LAYOUT_SLOT_MAJOR;
+  }
   h->cfg.layout = cfg.layout;
   h->n_nets = n_nets;
   auto fail = [&]() -> LeviathanHandle* {

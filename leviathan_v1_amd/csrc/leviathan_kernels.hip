@@ -106,14 +106,28 @@ __device__ __forceinline__ float row_coupling(const float* s_contrib, int tid, b
   return coupling;
 }
 
+// FP64 sums over a wavefront of the lanes' cos and sin of their rows' new
+// phases (lanes without a row hold +0.0).  A row's cosine is never zero (pi/2
+// is not a float), so a wavefront whose cosines are all +0.0 holds no row and
+// its sums are exactly +0.0: it skips the reductions.
+__device__ __forceinline__ void wave_sums_cs(double my_c, double my_s, double* wc, double* ws) {
+  if (__ballot(my_c != 0.0) == 0) {
+    *wc = 0.0;
+    *ws = 0.0;
+  } else {
+    *wc = wave_sum(my_c);
+    *ws = wave_sum(my_s);
+  }
+}
+
 // The block's FP64 partial of sum cos / sum sin of its new phases (lanes
 // without a row add zeros), summed in a fixed order: DPP wavefront sums, then
 // the four wavefronts in turn.
 __device__ __forceinline__ void store_block_partial(double my_c, double my_s, double (*s_red)[kBlock / 64],
                                                     double2* out) {
   const int tid = threadIdx.x;
-  const double wc = wave_sum(my_c);
-  const double ws = wave_sum(my_s);
+  double wc, ws;
+  wave_sums_cs(my_c, my_s, &wc, &ws);
   if ((tid & 63) == 0) {
     s_red[0][tid >> 6] = wc;
     s_red[1][tid >> 6] = ws;
@@ -479,15 +493,18 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 // Staged schedule, second pair (steps k, k + 1, k > 0) of a multi-step block:
 // an early edge's values of both steps straight from the ring, at slots
 // cur - delay and cur + 1 - delay (the step counter is at step k).  Step k's
-// value replaces the scatter's only for delay < k; step k + 1's always does.
+// value replaces the scatter's only for delay < k; for the others the first
+// load reads step k + 1's slot too (same line, no memory traffic of its own,
+// and no branch around a load) and its value is not used.
 template <class H, int LAYOUT>
 __device__ __forceinline__ void early_pair_values(const StepArgs& a, int2 rec, int32_t cur, H* v0, H* v1) {
   const H* __restrict__ ring = reinterpret_cast<const H*>(a.ring);
   const int32_t S = a.n_slots;
-  int32_t slot = cur - (rec.y & 3);
+  const int32_t d = rec.y & 3;
+  int32_t slot = cur - d;
   slot += (slot < 0) ? S : 0;
   const int32_t slot2 = slot + 1 == S ? 0 : slot + 1;
-  *v0 = ring[ring_index<LAYOUT>(a.ring_pitch, rec.x, slot)];
+  *v0 = ring[ring_index<LAYOUT>(a.ring_pitch, rec.x, d < a.early_k ? slot : slot2)];
   *v1 = ring[ring_index<LAYOUT>(a.ring_pitch, rec.x, slot2)];
 }
 
@@ -750,8 +767,8 @@ __global__ __launch_bounds__(kBlock) void pair_finish_kernel(StepArgs a) {
   // each block's partial as store_block_partial forms it: wavefront sums, then the four in turn
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
-    const double wc = wave_sum(my_c[j]);
-    const double ws = wave_sum(my_s[j]);
+    double wc, ws;
+    wave_sums_cs(my_c[j], my_s[j], &wc, &ws);
     if ((tid & 63) == 0) {
       s_red[j][0][tid >> 6] = wc;
       s_red[j][1][tid >> 6] = ws;

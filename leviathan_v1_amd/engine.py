@@ -48,8 +48,9 @@ class LeviathanEngine:
     Keyword extensions:
       history: "f32" (default) or "u16" (exact-wrap uint16 angles)
       layout:  "auto" (default: "node" for the direct schedule on a graph without delay-0
-               edges, where paired row passes read a node's two slots from one line; "slot"
-               otherwise), "slot" (ring[slot][node]) or "node" (ring[node][slot])
+               edges and with little index locality, where paired row passes read a node's
+               two slots from one line; "slot" otherwise), "slot" (ring[slot][node]) or
+               "node" (ring[node][slot])
       edges_per_thread: 4, 8 or 16 (stream-block capacity 256 * this)
       schedule: "direct" (one fused kernel, per-edge gather), "binned" (source-
                ordered scatter into destination buckets, then the fused row

@@ -8,8 +8,9 @@ so the bar here is exact equality of r, phases, weights, predictor, history and 
 row_steps = 1, for every schedule, both ring layouts and both history codings, on graphs with
   * rows beyond one block (long-row mode: two sweeps over the row, each lane reading back only
     the weights it stored),
-  * many short delays (delays 1..5: the early gathers of both steps of a pair run before it,
-    the second one a step ahead),
+  * many short delays (delays 1..5: many early edges, which the second pair of a staged block
+    gathers itself and the binned schedule gathers before the pair, the second step's a step
+    ahead),
   * a hub whose ~6,000 out-edges exceed one scatter block.
 Odd step counts leave single steps; 200 steps cross captured 100-step graphs.
 """
@@ -181,3 +182,25 @@ def test_row_steps_setting(ba50k):
     _same_state(a, ref, ba50k.max_delay)
     a.close()
     ref.close()
+
+
+def test_automatic_layout(ba50k, ws_d1):
+    """The default ring layout: node-major only for the direct schedule on a graph without
+    delay-0 edges and with little index locality (Barabasi-Albert); slot-major for a ring lattice
+    (most sources within 256 nodes), for delay-0 edges, and for two-phase schedules.  An explicit
+    layout is kept, and results do not depend on the layout."""
+    say("automatic layout")
+    ba0 = dataclasses.replace(ba50k, delays=np.where(np.arange(ba50k.E) % 97 == 0, 0, ba50k.delays)
+                              .astype(np.uint8))
+    cases = [(ba50k, {}, "node"), (ws_d1, {}, "slot"), (ba0, {}, "slot"),
+             (ba50k, dict(schedule="staged"), "slot"), (ba50k, dict(schedule="auto"), "slot"),
+             (ba50k, dict(layout="slot"), "slot"), (ws_d1, dict(layout="node"), "node")]
+    for wl, kw, want in cases:
+        e = make(wl, **kw)
+        assert e.layout == want, (wl.name, kw)
+        e.close()
+    a, b = make(ba50k), make(ba50k, layout="slot")
+    np.testing.assert_array_equal(a.step_n(DT, 30), b.step_n(DT, 30))
+    assert np.array_equal(a.get_theta(), b.get_theta()) and np.array_equal(a.get_weights(), b.get_weights())
+    a.close()
+    b.close()
