@@ -517,7 +517,10 @@ __device__ __forceinline__ void early_pair_values(const StepArgs& a, int2 rec, i
 // it needs the gain g(t + 1), which the finalize of step t computes from the
 // order parameter of all of step t's new phases.  The pass leaves each row's
 // step t + 1 coupling sum and prediction error in a.pair, and
-// pair_finish_kernel integrates after that finalize.
+// pair_finish_kernel integrates after that finalize.  The history slot of
+// step t's new phase is written by pair_finish_kernel too, with step t + 1's:
+// nothing reads it before the next pair (step t + 1 reads delays >= 1), and on
+// a node-major ring the two slots share a line, which then leaves the L2 once.
 //
 // In the second pair of a multi-step block of the staged schedule, the pass
 // also gathers its block's early edges (a.early_k > 0): the block patches
@@ -558,7 +561,6 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
   const int32_t e0 = it.e0;
   const int32_t nnz = it.nnz;
   const int32_t nrows = it.nrows;
-  const int64_t pitch = a.ring_pitch;
   float u_t = 0.0f;
   if (a.drive_b) u_t = a.drive_u[step % kDriveSteps];
 
@@ -628,8 +630,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     if (mine) {
       const int32_t i = r0 + tid;
       const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
-      a.theta[i] = x;
-      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      a.theta[i] = x;  // its history slot is written by the node pass (below)
       float sx, cx;
       sincos_fast(x, &sx, &cx);
       my_c = (double)cx;
@@ -683,8 +684,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     const float coupling = long_row_coupling(acc, s_red);
     if (tid == 0) {
       const float x = row_integrate(rs_, om, g, coupling, dt, a.alpha);
-      a.theta[i] = x;
-      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(pitch, i, nxt)] = HistCodec<H>::encode(x);
+      a.theta[i] = x;  // its history slot is written by the node pass (below)
       float sx, cx;
       sincos_fast(x, &sx, &cx);
       my_c = (double)cx;
@@ -709,7 +709,9 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
 
 // Second half of a paired row pass: step t + 1's integration of every row,
 // after the finalize of step t has set g(t + 1) and advanced the step counter,
-// from the coupling sum and prediction error pair_kernel left.  A workgroup
+// from the coupling sum and prediction error pair_kernel left.  It writes the
+// history slots of both steps' new phases (theta(t + 1), which pair_kernel
+// left in theta, and theta(t + 2)).  A workgroup
 // takes kFinishBlocks row blocks, with each block's rows on the lanes the
 // single-step kernel puts them on (a long row on lane 0), so each block
 // partial of r is the single-step kernel's to the bit.  All its loads are
@@ -723,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void pair_finish_kernel(StepArgs a) {
   const int32_t b0 = (int32_t)blockIdx.x * NB;
   const int64_t step = *a.step;
   const int32_t S = a.n_slots;
-  const int32_t cur = (int32_t)(step % S);
+  const int32_t cur = (int32_t)(step % S);  // the slot of theta(t + 1), the pass's starting phases
   const int32_t nxt = (cur + 1 == S) ? 0 : cur + 1;
   const float u_t = a.drive_b ? a.drive_u[step % kDriveSteps] : 0.0f;
   int32_t row[NB];
@@ -757,7 +759,9 @@ __global__ __launch_bounds__(kBlock) void pair_finish_kernel(StepArgs a) {
       rs.eps = ce[j].y;
       const float x = row_integrate(rs, om[j], g[j], ce[j].x, a.dt, a.alpha);
       a.theta[row[j]] = x;
-      reinterpret_cast<H*>(a.ring)[ring_index<LAYOUT>(a.ring_pitch, row[j], nxt)] = HistCodec<H>::encode(x);
+      H* const ring = reinterpret_cast<H*>(a.ring);
+      ring[ring_index<LAYOUT>(a.ring_pitch, row[j], cur)] = HistCodec<H>::encode(th[j]);
+      ring[ring_index<LAYOUT>(a.ring_pitch, row[j], nxt)] = HistCodec<H>::encode(x);
       float sx, cx;
       sincos_fast(x, &sx, &cx);
       my_c[j] = (double)cx;
