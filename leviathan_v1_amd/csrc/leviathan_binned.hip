@@ -139,10 +139,21 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         const int32_t r = id / CPR, c = id % CPR;
         ch[j] = *reinterpret_cast<const uint4*>(ring + (int64_t)r * pitch + w0 + c * EPC);
       }
+      // Ring slot r goes to window row (r - cur - 1) mod S.  Step t + q's value
+      // of an edge with delay d, ring slot cur + q - d, is then row S - 1 - d + q
+      // for every q <= d: an edge's K values lie in consecutive rows from
+      // S - 1 - d, with no wrap to compute.  For q > d the value does not exist
+      // yet (an early edge, gathered later into the array the row pass reads):
+      // its row is clamped to S - 1, inside the window.  (Per value this is an
+      // add and a minimum where the wrapped slot took about ten instructions.)
 #pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int32_t id = j * T + tid;
-        if (id < nch) *reinterpret_cast<uint4*>(s_win + (id / CPR) * kScatterWinCols + (id % CPR) * EPC) = ch[j];
+        if (id < nch) {
+          int32_t row = id / CPR - cur - 1;
+          row += (row < 0) ? S : 0;
+          *reinterpret_cast<uint4*>(s_win + row * kScatterWinCols + (id % CPR) * EPC) = ch[j];
+        }
       }
 #pragma unroll
       for (int j = 0; j < kScatterRuns / T; ++j) {
@@ -150,15 +161,15 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(ScatterArgs a) {
         if (r < nr) s_base[r] = tb[j];
       }
       __syncthreads();
-      const int32_t col0 = s0 - w0;
+      // the last window row at the block's first source
+      const int32_t top = (S - 1) * kScatterWinCols + (s0 - w0);
       V pv[EPT];
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
+      for (int k = 0; k < EPT; ++k) {
+        const int32_t last = top + (int32_t)((code[k] >> 8) & 255u);         // row S - 1, the edge's column
+        const int32_t first = last - (int32_t)(code[k] & 255u) * kScatterWinCols;  // row S - 1 - d
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-          const int32_t slot = ahead_slot(cur, q, (int32_t)(code[k] & 255u), S);
-          pv[k].x[q] = s_win[slot * kScatterWinCols + col0 + (int32_t)((code[k] >> 8) & 255u)];
-        }
+        for (int q = 0; q < K; ++q) pv[k].x[q] = s_win[min(first + q * kScatterWinCols, last)];
       }
       // the block's runs of consecutive slots: index le of run r goes to slot run_base[r] + le
       // (a lane past the block's edges loaded edge 0's code and stores nothing)
