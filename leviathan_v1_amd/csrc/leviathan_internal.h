@@ -66,13 +66,15 @@ struct StepArgs {
   const int64_t* step;    // global step counter (device)
   const float* drive_b;   // NULL: no input drive
   const float* drive_u;   // [kDriveSteps] ring, u_t at t % kDriveSteps
-  const void* hbuf;       // binned schedule: delayed phases written by the scatter kernel
+  // binned schedule: delayed phases written by the scatter kernel; staged: the regrouped values of
+  // the step (written here only by a paired pass gathering a long row's early edges)
+  void* hbuf;
   const uint32_t* hpos;   // binned schedule: [E] position of each CSR edge's value in hbuf
   int32_t hstride;        // binned schedule: elements per hbuf slot (the K interleaved steps of a scatter pass)
   const uint16_t* lpos;   // staged schedule: [E] position of each CSR edge's value in its block's range
   // paired row pass (two steps per pass): the staged values of the second step, and per node the
   // second step's coupling sum and prediction error, left for its integration
-  const void* hbuf_next;
+  void* hbuf_next;
   float2* pair;
   // staged schedule, the second pair of a multi-step block (early_k = its first step's index k > 0):
   // each row block's early edges, early[early_ptr[b] .. early_ptr[b + 1]) = {source, position in the

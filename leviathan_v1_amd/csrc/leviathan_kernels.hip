@@ -659,8 +659,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
       // second pair of a block: the row's early values go to its range of the staged arrays (CSR
       // order), which the sweeps read after the barrier below
       if (a.early_k > 0) {
-        H* const h0 = const_cast<H*>(reinterpret_cast<const H*>(a.hbuf));
-        H* const h1 = const_cast<H*>(reinterpret_cast<const H*>(a.hbuf_next));
+        H* const h0 = reinterpret_cast<H*>(a.hbuf);
+        H* const h1 = reinterpret_cast<H*>(a.hbuf_next);
         for (int32_t q = a.early_ptr[b] + tid; q < a.early_ptr[b + 1]; q += kBlock) {
           const int2 rec = a.early[q];
           H v0, v1;
