@@ -13,6 +13,12 @@
 //                    statistics handle is attached, the hub state of the
 //                    new phases.
 //
+// Paired row passes (graphs without delay-0 edges, leviathan_step_n): two
+// consecutive steps take pair_kernel (step t in full and step t+1's prep,
+// Hebbian update and coupling sum), finalize_kernel (step t), and
+// pair_finish_kernel (step t+1's integration with the gain the finalize just
+// set, and both steps' history slots), then finalize_kernel (step t+1).
+//
 // The fusion is legal because a row only writes its own state (theta,
 // theta_hat, omega_hat, its incoming edges' weights and its own history
 // slot), and the history ring has S = max_delay + 2 slots: the slot written
