@@ -118,15 +118,21 @@ def test_device_accumulate_ten_hubs_many_transitions():
         hip.hipFree(ptr)
 
 
-@pytest.mark.parametrize("history,schedule", [("f32", "direct"), ("u16", "staged")])
-def test_attached_engine_records_every_step(history, schedule):
+@pytest.mark.parametrize("history,schedule,K", [("f32", "direct", 1), ("u16", "staged", 1), ("u16", "staged", 4)])
+def test_attached_engine_records_every_step(history, schedule, K):
+    """The engine's step_n (paired row passes for the direct schedule and for four steps per scatter
+    pass: each pair's two finalizes record the phases of their own step) against single steps with
+    the reference's host round trip."""
     from leviathan_v1_amd.phi import top_degree_hubs
 
-    say("phi attached", history, schedule)
+    say("phi attached", history, schedule, K)
     wl = _wl()
     hubs = top_degree_hubs(wl.graph.row_ptr, wl.graph.col_idx, 6)
     a = _engine(wl, history=history, schedule=schedule)
     b = _engine(wl, history=history, schedule=schedule)
+    if K > 1:
+        a.set_scatter_steps(K)
+    assert a.row_steps == 2
     w = _worker(hubs)
     w.attach(a)
     o = P.PhiOracle(hubs)
