@@ -447,16 +447,20 @@ bool block_early_edges(const HostPlan& plan, const std::vector<int32_t>& src, co
   std::vector<int32_t> starts(n_items);
   for (size_t k = 0; k < n_items; ++k) starts[k] = plan.items[by_e0[k]].y;
   std::vector<int32_t> item_of(csr.size(), -1);
-  for (size_t r = 0; r < csr.size(); ++r) {
+  std::atomic<bool> ok{true};
+  parallel_for((int64_t)csr.size(), [&](int64_t r) {
     const int32_t e = (int32_t)csr[r];
     size_t k = (size_t)(std::upper_bound(starts.begin(), starts.end(), e) - starts.begin());
     // padding items hold no edges: step back over empty ranges to the item that holds e
     while (k > 0 && !(plan.items[by_e0[k - 1]].y <= e && e < plan.items[by_e0[k - 1]].z)) --k;
-    if (k == 0 || delay[r] > 3) {
-      err = "internal error: an early edge outside every row block, or a delay above 3";
-      return false;
-    }
-    item_of[r] = by_e0[k - 1];
+    if (k == 0 || delay[r] > 3)
+      ok.store(false);
+    else
+      item_of[r] = by_e0[k - 1];
+  });
+  if (!ok.load()) {
+    err = "internal error: an early edge outside every row block, or a delay above 3";
+    return false;
   }
   ptr.assign(n_items + 1, 0);
   for (int32_t q : item_of) ++ptr[(size_t)q + 1];
