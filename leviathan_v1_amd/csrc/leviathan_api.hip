@@ -448,10 +448,15 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
   }
   const bool staged = pl.kind == LEV_SCHEDULE_STAGED;
   const size_t hsize = h->cfg.history == LEV_HISTORY_F32 ? 4 : 2;
-  // the values step k's row pass reads: the regrouped array of step k (staged), or component k of
-  // the bucketed slots, which hold the K steps' values side by side (binned)
+  // uint16 values regrouped for paired row passes leave as pairs: step 2p's and 2p + 1's values of
+  // a staged slot side by side in pair array p, one 4-byte load for both steps of a pair
+  const bool packed = staged && pair && h->cfg.history == LEV_HISTORY_U16;
+  // the values step k's row pass reads: the regrouped array of step k (staged; packed: the first
+  // of its pair's values), or component k of the bucketed slots, which hold the K steps' values
+  // side by side (binned)
   auto values_of = [&](int kk) {
-    return staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + (size_t)kk * h->E * hsize)
+    const size_t at = packed ? (size_t)(kk / 2) * 2 * h->E + (size_t)(kk % 2) : (size_t)kk * h->E;
+    return staged ? static_cast<void*>(static_cast<char*>(h->d_hbuf2) + at * hsize)
                   : static_cast<void*>(static_cast<char*>(h->d_hbuf) + (size_t)kk * hsize);
   };
   void* hk = values_of(k);
@@ -490,6 +495,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
     ra.num_chunks = pl.n_rg_chunks;
     ra.stride = h->E;
     ra.steps = K;
+    ra.pack_pairs = packed ? 1 : 0;
     e = lev::launch_regroup(h->cfg.history, ra, h->stream);
   }
   if (e == hipSuccess && mid2) e = hipEventRecord(mid2, h->stream);
@@ -504,7 +510,7 @@ hipError_t enqueue_main(LeviathanHandle* h, float dt, int k, int K, hipEvent_t m
         a.early = pl.d_pe;
         a.early_k = k;
       }
-      e = lev::launch_pair(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
+      e = lev::launch_pair(h->cfg.history, h->cfg.layout, pl.ept, packed ? 3 : gather, a, h->stream);
     } else {
       e = lev::launch_step(h->cfg.history, h->cfg.layout, pl.ept, gather, a, h->stream);
     }

@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "leviathan_internal.h"
 #include "step_kernel.h"
 
@@ -481,29 +483,51 @@ __global__ __launch_bounds__(kBlock) void regroup_kernel(RegroupArgs a) {
   }
 }
 
+// One vector of a scatter pass's uint16 values (4 or 8 bytes) loaded with the
+// streaming (nontemporal) hint.
+template <class V>
+__device__ __forceinline__ V stream_load(const V* p) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "a 4- or 8-byte vector");
+  V v;
+  if constexpr (sizeof(V) == 4) {
+    const uint32_t raw = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+    __builtin_memcpy(&v, &raw, 4);
+  } else {
+    const uint64_t raw = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
+    __builtin_memcpy(&v, &raw, 8);
+  }
+  return v;
+}
+
 // Regroup of the values of K consecutive steps at once (several steps per
 // scatter pass): one chunk per workgroup.  Each bucket slot holds its edge's
-// K values side by side, so one load, one LDS placement and one LDS read move
-// all K; the rank, the run heads' scan and each position's staged slot are
-// computed once, and the K values leave to the K steps' staged arrays.
-template <class H, int K>
+// K values side by side, so one load moves all K; the rank, the run heads'
+// scan and each position's staged slot are computed once, and the K values
+// leave to the K steps' staged arrays.
+// PACK (uint16 values, paired row passes): the values of steps 2p and 2p + 1
+// leave together as one 4-byte pair to pair array p ([K / 2][stride] pairs),
+// which the paired row pass reads with one load per slot for both steps: K / 2
+// rounds of placement and stores instead of K.
+template <class H, int K, bool PACK>
 __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   using V = HVec<H, K>;
+  static_assert(!PACK || (sizeof(H) == 2 && K % 2 == 0), "pairs of uint16 values");
   constexpr int PER = kRegroupChunk / kBlock;
-  // The values are placed in LDS one step at a time through a buffer of
-  // single values, two barriers per step.  (Placing whole vectors, one LDS
-  // write and read per position, needs 32 KB per workgroup at two uint16
-  // steps: 4 instead of 6 workgroups per CU, and the pass took 0.71 instead of
-  // 0.61 ms at BA N = 10^7.)
-  constexpr bool kWhole = false;
-  __shared__ V s_vec[kWhole ? kRegroupChunk : 1];
-  __shared__ H s_one[kWhole ? 1 : kRegroupChunk];
+  // The values are placed in LDS one step (PACK: one pair of steps) at a time,
+  // two barriers per round.  (Placing whole vectors, one LDS write and read per
+  // position, needs 32 KB per workgroup at two uint16 steps: 4 instead of 6
+  // workgroups per CU, and the pass took 0.71 instead of 0.61 ms at BA N = 10^7.)
+  // PACK's 16 KB of pairs fill the 32 KB that five workgroups per CU leave each
+  // with the run heads and records, so the scan's wavefront sums borrow the
+  // pair buffer before the first round is placed.
+  using P = typename std::conditional<PACK, uint32_t, H>::type;
+  __shared__ P s_val[kRegroupChunk];          // one round's values (pairs) in staged-slot order
   __shared__ uint16_t s_run[kRegroupChunk];  // run heads, then each position's run
   __shared__ int2 s_runs[kRegroupRuns];
-  __shared__ int32_t s_wsum[kBlock / 64];
+  __shared__ int32_t s_wsum_own[PACK ? 1 : kBlock / 64];
+  int32_t* const s_wsum = PACK ? reinterpret_cast<int32_t*>(s_val) : s_wsum_own;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const V* __restrict__ src = reinterpret_cast<const V*>(a.src);
-  H* __restrict__ dst = reinterpret_cast<H*>(a.dst);
   // each XCD takes a contiguous run of chunks: the few chunks that fill one
   // staged line (consecutive source ranges of a bucket) meet in one L2.
   // Measured at BA N = 10^7, u16, two steps: 0.83 -> 0.72 ms, writes 1.56 ->
@@ -512,16 +536,34 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   const int4 ch = a.chunks[xcd_block(blockIdx.x, a.num_chunks, 1)];
   const int32_t n = ch.y, r0 = ch.z, nr = ch.w;
 
-  // unconditional loads: a lane past the chunk reads element 0 and discards it
+  // unconditional loads: a lane past the chunk reads element 0 and discards it.
+  // uint16 values: the chunk's values and ranks are read once, with the
+  // streaming hint, and then leave the L2 to the staged lines that neighbouring
+  // chunks fill in turn (BA N = 10^7, four steps per pass: 0.224 -> 0.203 ms
+  // per step).  FP32 values read without it (the hint: 0.492 -> 0.519), and the
+  // same hint on the scatter's window loads or the row pass's weight loads made
+  // those passes slower (uint16: 0.243 -> 0.353 and 0.395 -> 0.450).
   V v[PER];
   int32_t rk[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int32_t l = k * kBlock + tid;
     const int64_t c = l < n ? (int64_t)ch.x + l : 0;
-    rk[k] = a.rank[c];
-    v[k] = src[c];
+    if constexpr (sizeof(H) == 2) {
+      rk[k] = __builtin_nontemporal_load(a.rank + c);
+      v[k] = stream_load(src + c);
+    } else {
+      rk[k] = a.rank[c];
+      v[k] = src[c];
+    }
   }
+  // round q's value (PACK: pair) of position k
+  auto value = [&](int k, int q) -> P {
+    if constexpr (PACK)
+      return (uint32_t)v[k].x[2 * q] | ((uint32_t)v[k].x[2 * q + 1] << 16);
+    else
+      return v[k].x[q];
+  };
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int32_t l = k * kBlock + tid;
@@ -530,14 +572,11 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
   for (int32_t r = tid; r < nr; r += kBlock) s_runs[r] = a.runs[r0 + r];
   __syncthreads();
   for (int32_t r = tid; r < nr; r += kBlock) s_run[s_runs[r].x] = 1;
+  if constexpr (!PACK) {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int32_t l = k * kBlock + tid;
-    if (l < n) {
-      if constexpr (kWhole)
-        s_vec[rk[k]] = v[k];
-      else
-        s_one[rk[k]] = v[k].x[0];
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) s_val[rk[k]] = value(k, 0);
     }
   }
   __syncthreads();
@@ -567,9 +606,9 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
       s_run[l] = (uint16_t)(run - 1);
     }
   }
-  __syncthreads();
+  __syncthreads();  // (PACK: every lane has read the wavefront sums from the pair buffer)
 
-  // each position's staged slot, then the K steps' stores
+  // each position's staged slot, then the rounds' stores
   int32_t off[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -580,34 +619,22 @@ __global__ __launch_bounds__(kBlock) void regroup_multi_kernel(RegroupArgs a) {
       off[k] = rr.y + (l - rr.x);
     }
   }
-  if constexpr (kWhole) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int32_t l = k * kBlock + tid;
-      if (l < n) {
-        const V val = s_vec[l];
-#pragma unroll
-        for (int q = 0; q < K; ++q) dst[q * a.stride + off[k]] = val.x[q];
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      if (q > 0) {
-        __syncthreads();  // the previous step's values have left the LDS
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-          const int32_t l = k * kBlock + tid;
-          if (l < n) s_one[rk[k]] = v[k].x[q];
-        }
-        __syncthreads();
-      }
-      H* out = dst + q * a.stride;
+  for (int q = 0; q < (PACK ? K / 2 : K); ++q) {
+    if (q > 0 || PACK) {
+      if (q > 0) __syncthreads();  // the previous round's values have left the LDS
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int32_t l = k * kBlock + tid;
-        if (l < n) out[off[k]] = s_one[l];
+        if (l < n) s_val[rk[k]] = value(k, q);
       }
+      __syncthreads();
+    }
+    P* __restrict__ out = reinterpret_cast<P*>(a.dst) + q * a.stride;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int32_t l = k * kBlock + tid;
+      if (l < n) out[off[k]] = s_val[l];
     }
   }
 }
@@ -616,13 +643,16 @@ template <int K>
 static void regroup_multi_launch(int history, const RegroupArgs& a, hipStream_t s) {
   const dim3 grid(a.num_chunks), block(kBlock);
   if (history == 0)
-    regroup_multi_kernel<float, K><<<grid, block, 0, s>>>(a);
+    regroup_multi_kernel<float, K, false><<<grid, block, 0, s>>>(a);
+  else if (a.pack_pairs)
+    regroup_multi_kernel<uint16_t, K, true><<<grid, block, 0, s>>>(a);
   else
-    regroup_multi_kernel<uint16_t, K><<<grid, block, 0, s>>>(a);
+    regroup_multi_kernel<uint16_t, K, false><<<grid, block, 0, s>>>(a);
 }
 
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s) {
   if (a.num_chunks <= 0) return hipSuccess;
+  if (a.pack_pairs && (history == 0 || a.steps % 2 != 0)) return hipErrorInvalidValue;
   switch (a.steps) {
     case 1: {
       const dim3 grid((a.num_chunks + kRegroupChunksPerWg - 1) / kRegroupChunksPerWg), block(kBlock);

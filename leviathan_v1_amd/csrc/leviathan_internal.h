@@ -121,10 +121,11 @@ struct RegroupArgs {
   const uint16_t* rank;      // [E] bucket slot -> index in its chunk's staged order
   const int2* runs;          // {first index in the chunk's staged order, staged slot}
   const void* src;           // bucketed values, [E][steps] (a vector of the pass's steps per slot)
-  void* dst;                 // staged values, [steps][stride]
+  void* dst;                 // staged values, [steps][stride] (pack_pairs: [steps / 2][stride] pairs)
   int32_t num_chunks;
-  int64_t stride;            // elements between consecutive steps' staged arrays (steps > 1)
+  int64_t stride;            // slots between consecutive steps' (or pairs') staged arrays (steps > 1)
   int32_t steps;             // steps whose values one pass regroups (1, 2 or kMaxScatterSteps)
+  int32_t pack_pairs;        // uint16 values, paired row passes: steps 2p, 2p + 1 as one 4-byte pair
 };
 hipError_t launch_regroup(int history, const RegroupArgs& a, hipStream_t s);
 

@@ -54,6 +54,9 @@ namespace lev {
 // source.  The block reads that range with coalesced loads into LDS and picks
 // edge e's value at e0 + lpos[e] there; a long row's range is in CSR order and
 // is read straight.
+// GATHER = 3 (staged schedule, paired row pass, uint16 history): as 2, with the
+// pair's two steps' values side by side (one 4-byte pair per staged slot,
+// written so by the regroup), so each slot is one load for both steps.
 struct Item {
   int32_t r0, e0, nnz, nrows, net;
   bool is_long;
@@ -224,7 +227,7 @@ __device__ __forceinline__ CompSum long_row_sweep(const StepArgs& a, const H* hb
       } else if constexpr (GATHER == 1) {
         hr[k] = hb[(int64_t)(uint32_t)jj[k] * a.hstride];
       } else {
-        hr[k] = hb[(uint32_t)jj[k]];
+        hr[k] = hb[(uint32_t)jj[k] * (GATHER == 3 ? 2u : 1u)];  // packed pairs: every other value
       }
     }
 #pragma unroll
@@ -319,16 +322,28 @@ struct StreamEdges {
       }
       ww[k] = a.w[e];
     }
-    if constexpr (GATHER == 2) {
+    if constexpr (GATHER >= 2) {
       // the block's staged values go to LDS (s_v[v]) until the edge pass
       H sv[NV][EPT];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const H* __restrict__ src = reinterpret_cast<const H*>(v == 0 ? a.hbuf : a.hbuf_next);
+      if constexpr (GATHER == 3) {
+        static_assert(NV == 2 && sizeof(H) == 2, "packed pairs hold two uint16 values per slot");
+        const uint32_t* __restrict__ src = reinterpret_cast<const uint32_t*>(a.hbuf);
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
           const int32_t le = k * kBlock + tid;
-          sv[v][k] = src[le < nnz ? e0 + le : 0];
+          const uint32_t p = src[le < nnz ? e0 + le : 0];
+          sv[0][k] = (H)(p & 0xFFFFu);
+          sv[1][k] = (H)(p >> 16);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const H* __restrict__ src = reinterpret_cast<const H*>(v == 0 ? a.hbuf : a.hbuf_next);
+#pragma unroll
+          for (int k = 0; k < EPT; ++k) {
+            const int32_t le = k * kBlock + tid;
+            sv[v][k] = src[le < nnz ? e0 + le : 0];
+          }
         }
       }
 #pragma unroll
@@ -586,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
   __shared__ float s_gate[kBlock];
   __shared__ uint8_t s_erow[CAP];
   __shared__ float s_contrib[CAP];
-  __shared__ float s_next[GATHER == 2 ? CAP : 1];  // staged schedule: step t + 1's values
+  __shared__ float s_next[GATHER >= 2 ? CAP : 1];  // staged schedule: step t + 1's values
   __shared__ double s_red[2][kBlock / 64];
 
   const int tid = threadIdx.x;
@@ -635,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     int32_t eq = 0, eq1 = 0;
     int2 erec = make_int2(0, -1);
     H ev0{}, ev1{};
-    if constexpr (GATHER == 2) {
+    if constexpr (GATHER >= 2) {
       if (a.early_k > 0) {
         eq = a.early_ptr[b] + tid;
         eq1 = a.early_ptr[b + 1];
@@ -649,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     float* const s_v[2] = {s_contrib, s_next};
     ed.load(a, e0, nnz, cur, s_v);
     __syncthreads();
-    if constexpr (GATHER == 2) {
+    if constexpr (GATHER >= 2) {
       if (a.early_k > 0) {  // uniform: patch the staged copies once every staging store is done
         auto as_lds = [](H v) { return sizeof(H) == 2 ? (float)v : HistCodec<H>::decode(v); };
         while (eq < eq1) {  // the fetched record, then the lane's next one (a block with > kBlock)
@@ -701,9 +716,10 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
     const int32_t i = r0;
     RowState rs_, rs2;
     float om = 0.0f;
-    if constexpr (GATHER == 2) {
+    if constexpr (GATHER >= 2) {
       // second pair of a block: the row's early values go to its range of the staged arrays (CSR
-      // order), which the sweeps read after the barrier below
+      // order; packed pairs: every other value from each step's start), which the sweeps read
+      // after the barrier below
       if (a.early_k > 0) {
         H* const h0 = reinterpret_cast<H*>(a.hbuf);
         H* const h1 = reinterpret_cast<H*>(a.hbuf_next);
@@ -711,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
           const int2 rec = a.early[q];
           H v0, v1;
           early_pair_values<H, LAYOUT>(a, rec, cur, &v0, &v1);
-          const int32_t slot = e0 + (rec.y >> 2);
+          const int64_t slot = (int64_t)(e0 + (rec.y >> 2)) * (GATHER == 3 ? 2 : 1);
           if ((rec.y & 3) < a.early_k) h0[slot] = v0;
           h1[slot] = v1;
         }
@@ -742,7 +758,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(StepArgs a) {
       s_gate[0] = rs2.gate;
     }
     __syncthreads();
-    // step t + 1: the ring one slot later, the bucketed slot's next component, or the next staged array
+    // step t + 1: the ring one slot later, the bucketed slot's next component, or the next staged
+    // array (packed pairs: the value after step t's)
     const H* hb2 = GATHER == 0   ? reinterpret_cast<const H*>(a.ring)
                    : GATHER == 1 ? reinterpret_cast<const H*>(a.hbuf) + 1
                                  : reinterpret_cast<const H*>(a.hbuf_next);
@@ -1049,6 +1066,9 @@ static hipError_t launch_step_g(int ept, int gather, const StepArgs& a, hipStrea
       return launch_step_t<H, LAYOUT, 1, PAIR>(ept, a, s);
     case 2:
       return launch_step_t<H, LAYOUT, 2, PAIR>(ept, a, s);
+    case 3:  // packed pairs: the paired pass over uint16 history only
+      if constexpr (PAIR && sizeof(H) == 2) return launch_step_t<H, LAYOUT, 3, PAIR>(ept, a, s);
+      return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
@@ -1070,7 +1090,7 @@ hipError_t launch_step(int history, int layout, int ept, int gather, const StepA
 
 hipError_t launch_pair(int history, int layout, int ept, int gather, const StepArgs& a, hipStream_t s) {
   if (a.num_items <= 0) return hipSuccess;
-  if (!a.pair || (gather == 2 && !a.hbuf_next)) return hipErrorInvalidValue;
+  if (!a.pair || (gather >= 2 && !a.hbuf_next)) return hipErrorInvalidValue;
   return launch_step_p<true>(history, layout, ept, gather, a, s);
 }
 
