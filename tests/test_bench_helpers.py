@@ -51,7 +51,7 @@ def test_paired_headline_traffic():
     pair, _ = _bytes(rec, "pair_kernel<")
     fin, _ = _bytes(rec, "pair_finish_kernel<")
     sc, _ = _bytes(rec, "scatter_kernel<")
-    rg, _ = _bytes(rec, "regroup_multi_kernel<unsigned short, 4>")
+    rg, _ = _bytes(rec, "regroup_multi_kernel<unsigned short, 4")
     want = (pair + fin) / 2 + (sc + rg) / 4
     got = bench.load_traffic("ba10m", "u16", "slot", "staged/8", 4, True)
     assert got == pytest.approx(want, rel=1e-12)
@@ -64,7 +64,7 @@ def test_unpaired_multistep_traffic():
     rec = _records()["ba10m:u16:slot:staged/8:k4"]
     row, _ = _bytes(rec, "step_kernel<")
     sc, n_sc = _bytes(rec, "scatter_kernel<")
-    rg, _ = _bytes(rec, "regroup_multi_kernel<unsigned short, 4>")
+    rg, _ = _bytes(rec, "regroup_multi_kernel<unsigned short, 4")
     early, n_early = _bytes(rec, "early_kernel<")
     want = row + rg / 4 + (sc + early * n_early / n_sc) / 4
     assert bench.load_traffic("ba10m", "u16", "slot", "staged/8", 4) == pytest.approx(want, rel=1e-12)

@@ -5,7 +5,9 @@ in full and step t+1's prediction, Hebbian update and coupling sum (each weight 
 once for the two steps), and a node pass integrates step t+1 once step t's finalize has set the
 new gain.  Every value is computed from the same operands in the same order as two single steps,
 so the bar here is exact equality of r, phases, weights, predictor, history and gains against
-row_steps = 1, for every schedule, both ring layouts and both history codings, on graphs with
+row_steps = 1, for every schedule, both ring layouts and both history codings (the staged
+schedule with uint16 history regroups the values of each pair of steps side by side, one 4-byte
+pair per slot, where single steps get one array per step), on graphs with
   * rows beyond one block (long-row mode: two sweeps over the row, each lane reading back only
     the weights it stored),
   * many short delays (delays 1..5: many early edges, which the second pair of a staged block
